@@ -1,0 +1,162 @@
+/*
+ * ojmi.h -- C ABI of the MI355X (gfx950) single-spin-flip annealing engine.
+ *
+ * Drop-in boundary for OpenJij's single-spin-flip hot path:
+ *   Algorithm<SingleSpinFlip>::run over ClassicalIsing / TransverseIsing on
+ *   Dense / CSRSparse graphs, driven by a schedule list.
+ * Plain C: pointers and sizes only, no exceptions cross it.  Host buffers are
+ * borrowed for the duration of a call (copied in or out).  One context owns
+ * its device buffers and one HIP stream; distinct contexts may be used from
+ * distinct threads concurrently, one context must not.
+ *
+ * Problem layout (the reference's, include/openjij/graph/dense.hpp:117-140 and
+ * include/openjij/utility/eigen.hpp:66-81): an (n+1) x (n+1) matrix J_ext with
+ * J_ext[i][j] = J_ij (i != j < n), J_ext[i][n] = h_i, zero diagonal for i < n,
+ * J_ext[n][n] = 1.  Only the upper triangle (j >= i) is read, as the
+ * reference symmetrises with selfadjointView<Upper> (dense.hpp:139,
+ * csr_sparse.hpp:96-101).  Energy E(s) = sum_{i<j} J_ij s_i s_j + sum_i h_i s_i
+ * (dense.hpp:176-192 without the dummy-corner constant).
+ *
+ * Random stream: Philox4x32-10, key = seed, counter (i>>2, t, r, m), word i&3,
+ * for spin i, global sweep t, replica r, Trotter slice m.  A spin flips iff
+ * dE < log(U) * (-1/beta) (classical) or dE < -log(U) (Trotter, dE already
+ * dimensionless), which is the reference's Metropolis rule
+ * (include/openjij/updater/single_spin_flip.hpp:80-82, :240) in threshold form.
+ *
+ * Status codes map to the exceptions the reference raises through pybind11:
+ * OJMI_EINVAL -> ValueError, OJMI_ERANGE -> IndexError, others -> RuntimeError.
+ */
+#ifndef OJMI_H
+#define OJMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OJMI_OK 0
+#define OJMI_EINVAL 1   /* bad argument (std::invalid_argument)            */
+#define OJMI_ERANGE 2   /* size / index mismatch (std::out_of_range)       */
+#define OJMI_EDEVICE 3  /* HIP error or no device                          */
+#define OJMI_ENOMEM 4   /* device allocation failed                        */
+#define OJMI_ENOTSUP 5  /* shape not supported by the GPU kernels          */
+#define OJMI_ERUNTIME 6 /* invalid model data (std::runtime_error)         */
+
+/* arithmetic of the sweep */
+#define OJMI_PREC_AUTO 0  /* INT32 for integer couplings, else FP64          */
+#define OJMI_PREC_FP64 1  /* plain: FP64 couplings and fields               */
+#define OJMI_PREC_FP32 2  /* mixed: FP32 couplings/fields, FP64 energies    */
+#define OJMI_PREC_INT32 3 /* exact: int8/int32 couplings, int32 fields      */
+
+/* kernel family that runs a context (reported by ojmi_get_info) */
+#define OJMI_KERNEL_DENSE_WAVE 1   /* one wave64 per replica, fields in VGPRs   */
+#define OJMI_KERNEL_SQA_WAVE 2     /* one wave64 per Trotter slice              */
+#define OJMI_KERNEL_CSR_LEVEL 3    /* one wave64 per replica, level-parallel   */
+#define OJMI_KERNEL_DENSE_SHARED 4 /* replicas of a workgroup share J rows     */
+
+typedef struct ojmi_ctx ojmi_ctx;
+
+typedef struct ojmi_info {
+  int64_t num_spins;      /* n                                               */
+  int64_t num_replicas;   /* R                                               */
+  int64_t num_trotter;    /* M (1 for classical)                             */
+  int32_t precision;      /* resolved OJMI_PREC_*                            */
+  int32_t kernel;         /* OJMI_KERNEL_*                                   */
+  int32_t refresh_every;  /* FP32 path: sweeps between field refreshes       */
+  int32_t coupling_bytes; /* stored bytes per coupling (1, 4 or 8)           */
+  int64_t last_sweeps;    /* expanded sweeps of the last ojmi_run            */
+  int64_t last_attempts;  /* flip attempts of the last ojmi_run              */
+  double last_kernel_ms;  /* device time of the last ojmi_run (HIP events)   */
+  double last_refresh_ms; /* part of it spent in field refreshes             */
+} ojmi_info;
+
+/* Thread-local message of the last failing call on this thread. */
+const char* ojmi_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+int ojmi_device_count(int* count);
+
+/*
+ * Dense context.  Replaces the construction of
+ *   ClassicalIsing<Dense>   (include/openjij/system/classical_ising.hpp:60-69)
+ *   TransverseIsing<Dense>  (include/openjij/system/transverse_ising.hpp:70-119)
+ * for num_replicas independent copies.  num_trotter = 1 selects the classical
+ * system; num_trotter >= 2 the Suzuki-Trotter one with transverse field gamma
+ * (num_trotter = 0 or < 0 is rejected; the reference rejects M < 2 for the
+ * transverse system, transverse_ising.hpp:78-81).
+ * J_ext: (n+1)*(n+1) doubles, row-major.  Spins start at +1.
+ */
+int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int64_t num_trotter,
+                      double gamma, int32_t precision, ojmi_ctx** out);
+
+/*
+ * CSR context.  Replaces ClassicalIsing<CSRSparse> / TransverseIsing<CSRSparse>
+ * (include/openjij/system/classical_ising.hpp:204-212,
+ *  include/openjij/system/transverse_ising.hpp:441-459) built from
+ * graph::CSRSparse (include/openjij/graph/csr_sparse.hpp:96-101).
+ * rowptr has n+2 entries (n+1 rows), col/val have rowptr[n+1] entries.
+ */
+int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const double* val,
+                    int64_t num_replicas, int64_t num_trotter, double gamma, int32_t precision,
+                    ojmi_ctx** out);
+
+/*
+ * Tuning knobs (no reference counterpart).  Keys:
+ *   "refresh_every": FP32 path, sweeps between FP32 field refreshes (0 = fields
+ *                    are accumulated from the start of each run only)
+ *   "group":         replicas per workgroup sharing coupling rows (FP32 path)
+ */
+int ojmi_set_option(ojmi_ctx* ctx, const char* key, int64_t value);
+
+/*
+ * Spins: int8 +-1, [num_replicas][num_trotter][n].  Replaces reset_spins /
+ * the rw `spin` and `trotter_spins` attributes
+ * (include/openjij/system/classical_ising.hpp:76-80, openjij/declare.hpp:426,615).
+ */
+int ojmi_set_spins(ojmi_ctx* ctx, const int8_t* spins);
+int ojmi_get_spins(ojmi_ctx* ctx, int8_t* spins);
+
+/* Transverse field Gamma (the rw `gamma` attribute, openjij/declare.hpp:619). */
+int ojmi_set_gamma(ojmi_ctx* ctx, double gamma);
+
+/*
+ * Anneal every replica through the schedule.  Replaces
+ *   Algorithm<SingleSpinFlip>::run (include/openjij/algorithm/algorithm.hpp:26-56)
+ * as bound by Algorithm_SingleSpinFlip_run (openjij/declare.hpp:687-793):
+ * for each entry l, one_mc_step[l] sweeps at beta[l] (and s[l] for the
+ * transverse system; pass s = NULL for the classical one).  Sweeps are
+ * numbered sweep_offset, sweep_offset+1, ... in the random stream, so a run
+ * split in two calls with consecutive offsets equals one call.
+ * beta must be >= 0 (the reference sampler rejects beta < 0,
+ * openjij/sampler/sa_sampler.py:122-125).
+ */
+int ojmi_run(ojmi_ctx* ctx, const double* beta, const double* s, const int64_t* one_mc_step,
+             int64_t num_entries, uint64_t seed, uint64_t sweep_offset);
+
+/*
+ * get_solution (include/openjij/result/get_solution.hpp:39-84): [R][n] int8.
+ * Classical: the spins.  Transverse: the first Trotter slice of minimum
+ * energy, per replica.
+ */
+int ojmi_solution(ojmi_ctx* ctx, int8_t* out);
+
+/* FP64 energies E(s) of every replica and slice, [R][M], evaluated on the GPU. */
+int ojmi_energies(ojmi_ctx* ctx, double* out);
+
+/*
+ * FP32 path: max over replicas and spins of |F32 - F64| between the fields the
+ * sweep carried and fields recomputed in FP64 from the final spins (0 for the
+ * FP64 and INT32 paths, whose fields are exact or not kept).
+ */
+int ojmi_field_drift(ojmi_ctx* ctx, double* max_abs_drift);
+
+int ojmi_get_info(ojmi_ctx* ctx, ojmi_info* info);
+
+int ojmi_destroy(ojmi_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OJMI_H */

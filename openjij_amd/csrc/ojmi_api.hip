@@ -1,0 +1,790 @@
+// ojmi_api.hip -- the C ABI (include/ojmi.h): contexts, validation, layout
+// conversion, schedule expansion and kernel dispatch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ojmi.h"
+#include "ojmi_kernels.h"
+#include "ojmi_wave.h"
+
+using namespace ojmi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define OJMI_HIP(call)                                                               \
+  do {                                                                               \
+    hipError_t _e = (call);                                                          \
+    if (_e != hipSuccess)                                                            \
+      return fail(_e == hipErrorOutOfMemory ? OJMI_ENOMEM : OJMI_EDEVICE,            \
+                  std::string(#call) + ": " + hipGetErrorString(_e));                \
+  } while (0)
+
+template <typename T>
+int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? OJMI_ENOMEM : OJMI_EDEVICE,
+                std::string("hipMalloc: ") + hipGetErrorString(e));
+  return OJMI_OK;
+}
+
+int pow2_slots(int64_t n) {
+  int64_t need = (n + 63) / 64;
+  int ns = 1;
+  while (ns < need) ns <<= 1;
+  return ns;
+}
+
+}  // namespace
+
+struct ojmi_ctx {
+  int kind = 0;  // 0 dense, 1 csr
+  int64_t n = 0, R = 0, M = 1;
+  double gamma = 0;
+  int prec = OJMI_PREC_FP64;
+  int family = OJMI_KERNEL_DENSE_WAVE;
+  int ftype = FT_F64, jtype = JT_F64;
+  int NS = 1;
+  int64_t Npad = 64;
+  int sqa_ppw = 1, sqa_w = 1;
+  int csr_wpb = 1;
+  int refresh_every = 0;
+  int group = 1;
+
+  // kernel-layout couplings
+  void* dJ = nullptr;
+  void* dh = nullptr;
+  // FP64 natural padded couplings for energies (Npad x Npad) and h
+  double* dJ64 = nullptr;
+  double* dh64 = nullptr;
+  // CSR level-order data
+  int32_t *d_order = nullptr, *d_lvl = nullptr, *d_deg = nullptr, *d_ell = nullptr;
+  void* d_ellval = nullptr;
+  int nlev = 0, D = 0;
+
+  int8_t* dspins = nullptr;
+  double* dH = nullptr;    // [C][Npad] FP64 fields for energies
+  double* dE = nullptr;    // [C]
+  double* ddrift = nullptr;
+  void* dF = nullptr;      // [R][Npad] carried fields (FP32 path)
+  bool fields_valid = false;
+
+  double *dp0_d = nullptr, *dp1_d = nullptr;
+  float *dp0_f = nullptr, *dp1_f = nullptr;
+  int64_t Tcap = 0;
+
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+
+  int64_t last_T = 0, last_attempts = 0;
+  double last_ms = 0, last_refresh_ms = 0;
+
+  ~ojmi_ctx() {
+    if (dJ == dJ64) dJ = nullptr;
+    if (dh == dh64) dh = nullptr;
+    void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
+                    dH, dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+// Read the upper triangle of the dense (n+1)^2 layout: Jup[i][j] for j>i
+// mirrored, h from column n.
+int check_dense_input(int64_t n, const double* J_ext) {
+  const int64_t n1 = n + 1;
+  for (int64_t i = 0; i < n; ++i) {
+    if (J_ext[i * n1 + i] != 0.0)
+      return fail(OJMI_ERUNTIME, "The diagonal elements of interaction matrix must be zero.");
+    for (int64_t j = i; j < n1; ++j)
+      if (!std::isfinite(J_ext[i * n1 + j]))
+        return fail(OJMI_EINVAL, "interaction matrix holds a non-finite value");
+  }
+  return OJMI_OK;
+}
+
+bool all_integer(const std::vector<double>& v) {
+  for (double x : v)
+    if (x != std::floor(x) || std::fabs(x) > 1e9) return false;
+  return true;
+}
+
+int resolve_precision(int32_t req, bool integer, double max_abs_field, int* out) {
+  if (req == OJMI_PREC_AUTO) {
+    *out = (integer && 2.0 * max_abs_field < 2147483647.0) ? OJMI_PREC_INT32 : OJMI_PREC_FP64;
+    return OJMI_OK;
+  }
+  if (req == OJMI_PREC_INT32) {
+    if (!integer) return fail(OJMI_EINVAL, "INT32 precision needs integer couplings");
+    if (2.0 * max_abs_field >= 2147483647.0)
+      return fail(OJMI_EINVAL, "INT32 precision: local fields may overflow int32");
+    *out = req;
+    return OJMI_OK;
+  }
+  if (req == OJMI_PREC_FP64 || req == OJMI_PREC_FP32) {
+    *out = req;
+    return OJMI_OK;
+  }
+  return fail(OJMI_EINVAL, "unknown precision");
+}
+
+int common_init(ojmi_ctx* c) {
+  OJMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  OJMI_HIP(hipEventCreate(&c->e0));
+  OJMI_HIP(hipEventCreate(&c->e1));
+  int rc;
+  if ((rc = dalloc(&c->dspins, (size_t)(c->R * c->M * c->n))) != OJMI_OK) return rc;
+  std::vector<int8_t> ones((size_t)(c->R * c->M * c->n), 1);
+  OJMI_HIP(hipMemcpy(c->dspins, ones.data(), ones.size(), hipMemcpyHostToDevice));
+  if ((rc = dalloc(&c->dE, (size_t)(c->R * c->M))) != OJMI_OK) return rc;
+  if ((rc = dalloc(&c->ddrift, (size_t)(c->R * c->M))) != OJMI_OK) return rc;
+  return OJMI_OK;
+}
+
+// Upload the dense couplings in the kernel layout of (ftype, jtype).
+int upload_dense(ojmi_ctx* c, const std::vector<double>& Jn /* n x n symmetric */,
+                 const std::vector<double>& h) {
+  const int64_t n = c->n, Np = c->Npad;
+  const int NS = c->NS;
+  int rc;
+  // FP64 natural padded copy (energies, drift)
+  {
+    std::vector<double> J64((size_t)(Np * Np), 0.0), h64((size_t)Np, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      std::memcpy(&J64[i * Np], &Jn[i * n], n * sizeof(double));
+      h64[i] = h[i];
+    }
+    if ((rc = dalloc(&c->dJ64, J64.size())) != OJMI_OK) return rc;
+    if ((rc = dalloc(&c->dh64, h64.size())) != OJMI_OK) return rc;
+    OJMI_HIP(hipMemcpy(c->dJ64, J64.data(), J64.size() * 8, hipMemcpyHostToDevice));
+    OJMI_HIP(hipMemcpy(c->dh64, h64.data(), h64.size() * 8, hipMemcpyHostToDevice));
+  }
+  if (c->jtype == JT_F64) {
+    c->dJ = c->dJ64;
+    c->dh = c->dh64;
+    return OJMI_OK;
+  }
+  if (c->jtype == JT_F32) {
+    std::vector<float> Jf((size_t)(Np * Np), 0.0f), hf((size_t)Np, 0.0f);
+    for (int64_t i = 0; i < n; ++i) {
+      for (int64_t j = 0; j < n; ++j) Jf[i * Np + j] = (float)Jn[i * n + j];
+      hf[i] = (float)h[i];
+    }
+    float *dJ, *dh;
+    if ((rc = dalloc(&dJ, Jf.size())) != OJMI_OK) return rc;
+    c->dJ = dJ;
+    if ((rc = dalloc(&dh, hf.size())) != OJMI_OK) return rc;
+    c->dh = dh;
+    OJMI_HIP(hipMemcpy(dJ, Jf.data(), Jf.size() * 4, hipMemcpyHostToDevice));
+    OJMI_HIP(hipMemcpy(dh, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
+    return OJMI_OK;
+  }
+  // integer paths: h as int32
+  std::vector<int32_t> hi((size_t)Np, 0);
+  for (int64_t i = 0; i < n; ++i) hi[i] = (int32_t)h[i];
+  int32_t* dh;
+  if ((rc = dalloc(&dh, hi.size())) != OJMI_OK) return rc;
+  c->dh = dh;
+  OJMI_HIP(hipMemcpy(dh, hi.data(), hi.size() * 4, hipMemcpyHostToDevice));
+  if (c->jtype == JT_I32) {
+    std::vector<int32_t> Ji((size_t)(Np * Np), 0);
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t j = 0; j < n; ++j) Ji[i * Np + j] = (int32_t)Jn[i * n + j];
+    int32_t* dJ;
+    if ((rc = dalloc(&dJ, Ji.size())) != OJMI_OK) return rc;
+    c->dJ = dJ;
+    OJMI_HIP(hipMemcpy(dJ, Ji.data(), Ji.size() * 4, hipMemcpyHostToDevice));
+    return OJMI_OK;
+  }
+  // int8 packed: element (m, l) of row i at byte ((m>>2)*64 + l)*4 + (m&3)
+  const int64_t rb = ((NS + 3) / 4) * 256;
+  std::vector<int8_t> J8((size_t)(Np * rb), 0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < n; ++j) {
+      const int64_t m = j >> 6, l = j & 63;
+      J8[i * rb + ((m >> 2) * 64 + l) * 4 + (m & 3)] = (int8_t)Jn[i * n + j];
+    }
+  int8_t* dJ;
+  if ((rc = dalloc(&dJ, J8.size())) != OJMI_OK) return rc;
+  c->dJ = dJ;
+  OJMI_HIP(hipMemcpy(dJ, J8.data(), J8.size(), hipMemcpyHostToDevice));
+  return OJMI_OK;
+}
+
+int setup_dense_family(ojmi_ctx* c) {
+  const int64_t n = c->n;
+  c->NS = pow2_slots(n);
+  c->Npad = 64LL * c->NS;
+  if (c->M == 1) {
+    if (c->NS > 64)
+      return fail(OJMI_ENOTSUP, "dense GPU path supports n <= 4096 spins");
+    c->family = OJMI_KERNEL_DENSE_WAVE;
+  } else {
+    const int64_t P = (c->M + 1) / 2;
+    const int words = (c->ftype == FT_F64) ? 2 : 1;
+    if (c->NS > 32) return fail(OJMI_ENOTSUP, "Trotter GPU path supports n <= 2048 spins");
+    auto pick = [&](int wmax, int ppw_max, int budget, int* Wout, int* Pout) -> bool {
+      const int W = (int)std::min<int64_t>(P, wmax);
+      const int64_t need = (P + W - 1) / W;
+      int PPW = 1;
+      while (PPW < need) PPW <<= 1;
+      if (PPW > ppw_max || 2 * PPW * c->NS * words > budget) return false;
+      *Wout = W;
+      *Pout = PPW;
+      return true;
+    };
+    int W = 0, PPW = 0;
+    // 16 waves per replica when the fields fit 128 VGPRs, else 4 waves with 512
+    if (!(P > 4 && pick(16, 2, 48, &W, &PPW)) && !pick(4, 8, 320, &W, &PPW))
+      return fail(OJMI_ENOTSUP, "Trotter GPU path: too many spins x slices per replica");
+    c->sqa_ppw = PPW;
+    c->sqa_w = W;
+    c->family = OJMI_KERNEL_SQA_WAVE;
+  }
+  return OJMI_OK;
+}
+
+void pick_types(ojmi_ctx* c, double maxabsJ) {
+  if (c->prec == OJMI_PREC_FP64) {
+    c->ftype = FT_F64;
+    c->jtype = JT_F64;
+  } else if (c->prec == OJMI_PREC_FP32) {
+    c->ftype = FT_F32;
+    c->jtype = JT_F32;
+  } else {
+    c->ftype = FT_I32;
+    c->jtype = (maxabsJ <= 127.0) ? JT_I8 : JT_I32;
+  }
+}
+
+int ensure_sched(ojmi_ctx* c, int64_t T) {
+  if (T <= c->Tcap) return OJMI_OK;
+  void* b[] = {c->dp0_d, c->dp1_d, c->dp0_f, c->dp1_f};
+  for (void* p : b)
+    if (p) (void)hipFree(p);
+  c->dp0_d = c->dp1_d = nullptr;
+  c->dp0_f = c->dp1_f = nullptr;
+  int rc;
+  if ((rc = dalloc(&c->dp0_d, (size_t)T)) != OJMI_OK) return rc;
+  if ((rc = dalloc(&c->dp1_d, (size_t)T)) != OJMI_OK) return rc;
+  if ((rc = dalloc(&c->dp0_f, (size_t)T)) != OJMI_OK) return rc;
+  if ((rc = dalloc(&c->dp1_f, (size_t)T)) != OJMI_OK) return rc;
+  c->Tcap = T;
+  return OJMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ojmi_last_error(void) { return g_err.c_str(); }
+
+int ojmi_device_count(int* count) {
+  if (!count) return fail(OJMI_EINVAL, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *count = (e == hipSuccess) ? c : 0;
+  return OJMI_OK;
+}
+
+int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int64_t num_trotter,
+                      double gamma, int32_t precision, ojmi_ctx** out) {
+  if (!out) return fail(OJMI_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (n < 1 || !J_ext) return fail(OJMI_EINVAL, "need n >= 1 and a coupling matrix");
+  if (num_replicas < 1) return fail(OJMI_EINVAL, "num_replicas must be >= 1");
+  if (num_trotter < 1) return fail(OJMI_EINVAL, "num_trotter must be >= 1");
+  int rc = check_dense_input(n, J_ext);
+  if (rc) return rc;
+  const int64_t n1 = n + 1;
+  std::vector<double> Jn((size_t)(n * n), 0.0), h((size_t)n);
+  double maxabsJ = 0, maxfield = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int64_t j = i + 1; j < n; ++j) {
+      const double v = J_ext[i * n1 + j];
+      Jn[i * n + j] = v;
+      Jn[j * n + i] = v;
+    }
+    h[i] = J_ext[i * n1 + n];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    double rs = std::fabs(h[i]);
+    for (int64_t j = 0; j < n; ++j) {
+      rs += std::fabs(Jn[i * n + j]);
+      maxabsJ = std::max(maxabsJ, std::fabs(Jn[i * n + j]));
+    }
+    maxfield = std::max(maxfield, rs);
+  }
+  const bool integer = all_integer(Jn) && all_integer(h);
+  int prec;
+  if ((rc = resolve_precision(precision, integer, maxfield, &prec))) return rc;
+  if (num_trotter > 1 && prec == OJMI_PREC_INT32) prec = OJMI_PREC_FP64;
+
+  ojmi_ctx* c = new (std::nothrow) ojmi_ctx;
+  if (!c) return fail(OJMI_ENOMEM, "host allocation failed");
+  c->kind = 0;
+  c->n = n;
+  c->R = num_replicas;
+  c->M = num_trotter;
+  c->gamma = gamma;
+  c->prec = prec;
+  pick_types(c, maxabsJ);
+  if ((rc = setup_dense_family(c)) || (rc = common_init(c)) || (rc = upload_dense(c, Jn, h))) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return OJMI_OK;
+}
+
+int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const double* val,
+                    int64_t num_replicas, int64_t num_trotter, double gamma, int32_t precision,
+                    ojmi_ctx** out) {
+  if (!out) return fail(OJMI_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (n < 1 || !rowptr || !col || !val)
+    return fail(OJMI_EINVAL, "need n >= 1 and CSR arrays");
+  if (num_replicas < 1) return fail(OJMI_EINVAL, "num_replicas must be >= 1");
+  if (num_trotter < 1) return fail(OJMI_EINVAL, "num_trotter must be >= 1");
+  if (rowptr[0] != 0) return fail(OJMI_EINVAL, "rowptr[0] must be 0");
+  for (int64_t i = 0; i <= n; ++i)
+    if (rowptr[i + 1] < rowptr[i]) return fail(OJMI_EINVAL, "rowptr must be non-decreasing");
+  // upper triangle -> symmetric adjacency without diagonal / dummy
+  std::vector<std::vector<std::pair<int64_t, double>>> adj((size_t)n);
+  std::vector<double> h((size_t)n, 0.0);
+  for (int64_t i = 0; i <= n; ++i)
+    for (int64_t e = rowptr[i]; e < rowptr[i + 1]; ++e) {
+      const int64_t j = col[e];
+      const double v = val[e];
+      if (j < 0 || j > n) return fail(OJMI_ERANGE, "column index out of range");
+      if (!std::isfinite(v)) return fail(OJMI_EINVAL, "non-finite coupling");
+      if (j < i || i == n) continue;
+      if (j == i) {
+        if (v != 0.0)
+          return fail(OJMI_ERUNTIME,
+                      "The diagonal elements of interaction matrix must be zero.");
+        continue;
+      }
+      if (j == n) {
+        h[i] += v;
+        continue;
+      }
+      if (v == 0.0) continue;
+      adj[i].push_back({j, v});
+      adj[j].push_back({i, v});
+    }
+  double maxabsJ = 0, maxfield = 0;
+  bool integer = all_integer(h);
+  int64_t D = 0, nnz = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    auto& a = adj[i];
+    std::sort(a.begin(), a.end(),
+              [](const std::pair<int64_t, double>& x, const std::pair<int64_t, double>& y) {
+                return x.first < y.first;
+              });
+    // merge duplicates (an upper-triangle input may list a pair twice)
+    std::vector<std::pair<int64_t, double>> m;
+    for (auto& p : a) {
+      if (!m.empty() && m.back().first == p.first) m.back().second += p.second;
+      else m.push_back(p);
+    }
+    a.swap(m);
+    double rs = std::fabs(h[i]);
+    for (auto& p : a) {
+      rs += std::fabs(p.second);
+      maxabsJ = std::max(maxabsJ, std::fabs(p.second));
+      if (p.second != std::floor(p.second) || std::fabs(p.second) > 1e9) integer = false;
+    }
+    maxfield = std::max(maxfield, rs);
+    D = std::max<int64_t>(D, (int64_t)a.size());
+    nnz += (int64_t)a.size();
+  }
+  int rc, prec;
+  if ((rc = resolve_precision(precision, integer, maxfield, &prec))) return rc;
+  if (num_trotter > 1 && prec == OJMI_PREC_INT32) prec = OJMI_PREC_FP64;
+
+  ojmi_ctx* c = new (std::nothrow) ojmi_ctx;
+  if (!c) return fail(OJMI_ENOMEM, "host allocation failed");
+  c->kind = 1;
+  c->n = n;
+  c->R = num_replicas;
+  c->M = num_trotter;
+  c->gamma = gamma;
+  c->prec = prec;
+  pick_types(c, maxabsJ);
+  if (c->jtype == JT_I8) c->jtype = JT_I32;
+
+  // Dense-ish graphs (or Trotter systems) run on the dense wave kernels.
+  const bool dense_like = (num_trotter > 1) || (D > 64 && 2 * nnz > n * (int64_t)64);
+  if (dense_like) {
+    std::vector<double> Jn((size_t)(n * n), 0.0);
+    for (int64_t i = 0; i < n; ++i)
+      for (auto& p : adj[i]) Jn[i * n + p.first] = p.second;
+    if (c->prec == OJMI_PREC_INT32) c->jtype = (maxabsJ <= 127.0) ? JT_I8 : JT_I32;
+    if ((rc = setup_dense_family(c)) || (rc = common_init(c)) ||
+        (rc = upload_dense(c, Jn, h))) {
+      delete c;
+      return rc;
+    }
+    *out = c;
+    return OJMI_OK;
+  }
+
+  // Level order: level(i) = 1 + max level of smaller-index neighbours.
+  std::vector<int32_t> level((size_t)n, 0);
+  int32_t nlev = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t lv = 0;
+    for (auto& p : adj[i])
+      if (p.first < i) lv = std::max(lv, level[p.first] + 1);
+    level[i] = lv;
+    nlev = std::max(nlev, lv + 1);
+  }
+  std::vector<int32_t> order((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return level[a] < level[b]; });
+  std::vector<int32_t> lvl_ptr((size_t)nlev + 1, 0);
+  for (int64_t i = 0; i < n; ++i) lvl_ptr[level[i] + 1]++;
+  for (int32_t l = 0; l < nlev; ++l) lvl_ptr[l + 1] += lvl_ptr[l];
+  if (D == 0) D = 1;
+  std::vector<int32_t> deg((size_t)n), ell((size_t)(D * n), 0);
+  std::vector<double> ellv((size_t)(D * n), 0.0), hpos((size_t)n);
+  for (int64_t pos = 0; pos < n; ++pos) {
+    const int64_t i = order[pos];
+    deg[pos] = (int32_t)adj[i].size();
+    hpos[pos] = h[i];
+    for (size_t d = 0; d < adj[i].size(); ++d) {
+      ell[d * n + pos] = (int32_t)adj[i][d].first;
+      ellv[d * n + pos] = adj[i][d].second;
+    }
+  }
+  c->family = OJMI_KERNEL_CSR_LEVEL;
+  c->nlev = nlev;
+  c->D = (int)D;
+  c->NS = pow2_slots(n);
+  c->Npad = 64LL * c->NS;
+  // per wave: Philox words (16 B per 4 spins) + int8 spins; <= 64 KiB per block
+  const size_t per_wave = (size_t)((n + 3) / 4) * 20;
+  c->csr_wpb = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per_wave));
+  if (per_wave > 65536) {
+    delete c;
+    return fail(OJMI_ENOTSUP, "CSR GPU path: n too large for LDS-resident spins");
+  }
+  if ((rc = common_init(c))) {
+    delete c;
+    return rc;
+  }
+  auto up32 = [&](int32_t** d, const std::vector<int32_t>& v) -> int {
+    int r = dalloc(d, v.size());
+    if (r) return r;
+    OJMI_HIP(hipMemcpy(*d, v.data(), v.size() * 4, hipMemcpyHostToDevice));
+    return OJMI_OK;
+  };
+  if ((rc = up32(&c->d_order, order)) || (rc = up32(&c->d_lvl, lvl_ptr)) ||
+      (rc = up32(&c->d_deg, deg)) || (rc = up32(&c->d_ell, ell))) {
+    delete c;
+    return rc;
+  }
+  // values and h at level-order position, in the field type
+  size_t esz = (c->ftype == FT_F64) ? 8 : 4;
+  std::vector<char> vb(ellv.size() * esz), hb(hpos.size() * esz);
+  for (size_t k = 0; k < ellv.size(); ++k) {
+    if (c->ftype == FT_F64) ((double*)vb.data())[k] = ellv[k];
+    else if (c->ftype == FT_F32) ((float*)vb.data())[k] = (float)ellv[k];
+    else ((int32_t*)vb.data())[k] = (int32_t)ellv[k];
+  }
+  for (size_t k = 0; k < hpos.size(); ++k) {
+    if (c->ftype == FT_F64) ((double*)hb.data())[k] = hpos[k];
+    else if (c->ftype == FT_F32) ((float*)hb.data())[k] = (float)hpos[k];
+    else ((int32_t*)hb.data())[k] = (int32_t)hpos[k];
+  }
+  char *dv, *dhb;
+  if ((rc = dalloc(&dv, vb.size())) || (rc = dalloc(&dhb, hb.size()))) {
+    delete c;
+    return rc;
+  }
+  c->d_ellval = dv;
+  c->dh = dhb;
+  if (hipMemcpy(dv, vb.data(), vb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dhb, hb.data(), hb.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    delete c;
+    return fail(OJMI_EDEVICE, "hipMemcpy of CSR values failed");
+  }
+  // FP64 dense copy for energies when it fits (n <= 16384), else CSR energies
+  if (n <= 16384) {
+    const int64_t Np = c->Npad;
+    std::vector<double> J64((size_t)(Np * Np), 0.0), h64((size_t)Np, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      for (auto& p : adj[i]) J64[i * Np + p.first] = p.second;
+      h64[i] = h[i];
+    }
+    if ((rc = dalloc(&c->dJ64, J64.size())) || (rc = dalloc(&c->dh64, h64.size()))) {
+      delete c;
+      return rc;
+    }
+    if (hipMemcpy(c->dJ64, J64.data(), J64.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->dh64, h64.data(), h64.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+      delete c;
+      return fail(OJMI_EDEVICE, "hipMemcpy of FP64 couplings failed");
+    }
+  }
+  *out = c;
+  return OJMI_OK;
+}
+
+int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
+  if (!c || !key) return fail(OJMI_EINVAL, "null argument");
+  const std::string k(key);
+  if (k == "refresh_every") {
+    if (value < 0) return fail(OJMI_EINVAL, "refresh_every must be >= 0");
+    c->refresh_every = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "group") {
+    if (value < 1 || value > 16) return fail(OJMI_EINVAL, "group must be in 1..16");
+    c->group = (int)value;
+    return OJMI_OK;
+  }
+  return fail(OJMI_EINVAL, "unknown option: " + k);
+}
+
+int ojmi_set_spins(ojmi_ctx* c, const int8_t* spins) {
+  if (!c || !spins) return fail(OJMI_EINVAL, "null argument");
+  const size_t cnt = (size_t)(c->R * c->M * c->n);
+  for (size_t k = 0; k < cnt; ++k)
+    if (spins[k] != 1 && spins[k] != -1)
+      return fail(OJMI_EINVAL, "spins must be +1 or -1");
+  OJMI_HIP(hipMemcpy(c->dspins, spins, cnt, hipMemcpyHostToDevice));
+  c->fields_valid = false;
+  return OJMI_OK;
+}
+
+int ojmi_get_spins(ojmi_ctx* c, int8_t* spins) {
+  if (!c || !spins) return fail(OJMI_EINVAL, "null argument");
+  OJMI_HIP(hipStreamSynchronize(c->st));
+  OJMI_HIP(hipMemcpy(spins, c->dspins, (size_t)(c->R * c->M * c->n), hipMemcpyDeviceToHost));
+  return OJMI_OK;
+}
+
+int ojmi_set_gamma(ojmi_ctx* c, double gamma) {
+  if (!c) return fail(OJMI_EINVAL, "null context");
+  c->gamma = gamma;
+  return OJMI_OK;
+}
+
+int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mcs, int64_t L,
+             uint64_t seed, uint64_t sweep_offset) {
+  if (!c) return fail(OJMI_EINVAL, "null context");
+  if (L < 0 || (L > 0 && (!beta || !mcs))) return fail(OJMI_EINVAL, "bad schedule arrays");
+  const bool transverse = c->M > 1;
+  if (transverse && L > 0 && !s) return fail(OJMI_EINVAL, "transverse system needs s[]");
+  int64_t T = 0;
+  for (int64_t l = 0; l < L; ++l) {
+    if (mcs[l] < 0) return fail(OJMI_EINVAL, "one_mc_step must be >= 0");
+    if (!(beta[l] >= 0.0)) return fail(OJMI_EINVAL, "schedule beta range is '0 <= beta'.");
+    T += mcs[l];
+  }
+  if (sweep_offset + (uint64_t)T > 0xffffffffull)
+    return fail(OJMI_EINVAL, "global sweep index exceeds 32 bits");
+  c->last_T = T;
+  c->last_attempts = T * c->R * c->M * c->n;
+  c->last_ms = 0;
+  c->last_refresh_ms = 0;
+  if (T == 0) return OJMI_OK;
+  int rc;
+  if ((rc = ensure_sched(c, T))) return rc;
+  std::vector<double> p0((size_t)T), p1((size_t)T, 0.0);
+  std::vector<float> p0f((size_t)T), p1f((size_t)T);
+  {
+    int64_t t = 0;
+    const double Md = (double)c->M;
+    for (int64_t l = 0; l < L; ++l) {
+      double a0, a1 = 0.0;
+      if (!transverse) {
+        a0 = -1.0 / beta[l];
+      } else {
+        // include/openjij/updater/single_spin_flip.hpp:147-148,216-217
+        a0 = s[l] * (beta[l] / Md);
+        a1 = (1 / 2.) * std::log(std::tanh(beta[l] * c->gamma * (1.0 - s[l]) / Md));
+      }
+      for (int64_t k = 0; k < mcs[l]; ++k, ++t) {
+        p0[t] = a0;
+        p1[t] = a1;
+      }
+    }
+    for (int64_t k = 0; k < T; ++k) {
+      p0f[k] = (float)p0[k];
+      p1f[k] = (float)p1[k];
+    }
+  }
+  OJMI_HIP(hipMemcpyAsync(c->dp0_d, p0.data(), T * 8, hipMemcpyHostToDevice, c->st));
+  OJMI_HIP(hipMemcpyAsync(c->dp1_d, p1.data(), T * 8, hipMemcpyHostToDevice, c->st));
+  OJMI_HIP(hipMemcpyAsync(c->dp0_f, p0f.data(), T * 4, hipMemcpyHostToDevice, c->st));
+  OJMI_HIP(hipMemcpyAsync(c->dp1_f, p1f.data(), T * 4, hipMemcpyHostToDevice, c->st));
+
+  OJMI_HIP(hipEventRecord(c->e0, c->st));
+  hipError_t e = hipSuccess;
+  if (c->family == OJMI_KERNEL_CSR_LEVEL) {
+    CsrArgs a{};
+    a.order = c->d_order;
+    a.lvl_ptr = c->d_lvl;
+    a.nlev = c->nlev;
+    a.deg = c->d_deg;
+    a.ell_col = c->d_ell;
+    a.val = c->d_ellval;
+    a.D = c->D;
+    a.h = c->dh;
+    a.spins = c->dspins;
+    a.n = (int)c->n;
+    a.R = (int)c->R;
+    a.M = 1;
+    a.T = (int)T;
+    a.seed = seed;
+    a.t0 = sweep_offset;
+    a.rep_offset = 0;
+    a.p0_d = c->dp0_d;
+    a.p0_f = c->dp0_f;
+    e = launch_csr_level(c->ftype, a, c->csr_wpb, c->st);
+  } else {
+    DenseArgs a{};
+    a.J = c->dJ;
+    a.h = c->dh;
+    a.spins = c->dspins;
+    a.n = (int)c->n;
+    a.R = (int)c->R;
+    a.M = (int)c->M;
+    a.T = (int)T;
+    a.seed = seed;
+    a.t0 = sweep_offset;
+    a.rep_offset = 0;
+    a.p0_d = c->dp0_d;
+    a.p0_f = c->dp0_f;
+    a.p1_d = c->dp1_d;
+    a.p1_f = c->dp1_f;
+    if (c->family == OJMI_KERNEL_SQA_WAVE) {
+      e = launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
+    } else {
+      if (c->ftype == FT_F32) {
+        if (!c->dF) {
+          float* f;
+          if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
+          c->dF = f;
+        }
+        a.F_out = c->dF;
+      }
+      e = launch_dense_wave(c->ftype, c->jtype, a, c->NS, c->st);
+      c->fields_valid = (c->ftype == FT_F32);
+    }
+  }
+  if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  OJMI_HIP(hipEventRecord(c->e1, c->st));
+  OJMI_HIP(hipEventSynchronize(c->e1));
+  float ms = 0;
+  OJMI_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+  c->last_ms = ms;
+  return OJMI_OK;
+}
+
+static int compute_energies(ojmi_ctx* c) {
+  const int64_t C = c->R * c->M;
+  if (!c->dJ64) return fail(OJMI_ENOTSUP, "energies need the FP64 coupling copy");
+  int rc;
+  if (!c->dH && (rc = dalloc(&c->dH, (size_t)(C * c->Npad)))) return rc;
+  const float* F32 = (c->fields_valid && c->dF && c->M == 1) ? (const float*)c->dF : nullptr;
+  hipError_t e = launch_energy_f64(c->dJ64, c->dh64, c->dspins, (int)c->n, (int)c->Npad, (int)C,
+                                   c->dH, c->dE, F32, F32 ? c->ddrift : nullptr, c->st);
+  if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("energy kernel: ") + hipGetErrorString(e));
+  OJMI_HIP(hipStreamSynchronize(c->st));
+  return OJMI_OK;
+}
+
+int ojmi_energies(ojmi_ctx* c, double* out) {
+  if (!c || !out) return fail(OJMI_EINVAL, "null argument");
+  int rc = compute_energies(c);
+  if (rc) return rc;
+  OJMI_HIP(hipMemcpy(out, c->dE, (size_t)(c->R * c->M) * 8, hipMemcpyDeviceToHost));
+  return OJMI_OK;
+}
+
+int ojmi_field_drift(ojmi_ctx* c, double* out) {
+  if (!c || !out) return fail(OJMI_EINVAL, "null argument");
+  *out = 0.0;
+  if (!(c->fields_valid && c->dF && c->M == 1)) return OJMI_OK;
+  int rc = compute_energies(c);
+  if (rc) return rc;
+  std::vector<double> d((size_t)c->R);
+  OJMI_HIP(hipMemcpy(d.data(), c->ddrift, d.size() * 8, hipMemcpyDeviceToHost));
+  double m = 0;
+  for (double x : d) m = std::max(m, x);
+  *out = m;
+  return OJMI_OK;
+}
+
+int ojmi_solution(ojmi_ctx* c, int8_t* out) {
+  if (!c || !out) return fail(OJMI_EINVAL, "null argument");
+  const int64_t n = c->n, M = c->M, R = c->R;
+  std::vector<int8_t> sp((size_t)(R * M * n));
+  OJMI_HIP(hipStreamSynchronize(c->st));
+  OJMI_HIP(hipMemcpy(sp.data(), c->dspins, sp.size(), hipMemcpyDeviceToHost));
+  if (M == 1) {
+    std::memcpy(out, sp.data(), sp.size());
+    return OJMI_OK;
+  }
+  std::vector<double> E((size_t)(R * M));
+  int rc = ojmi_energies(c, E.data());
+  if (rc) return rc;
+  for (int64_t r = 0; r < R; ++r) {
+    int64_t tb = 0;
+    for (int64_t t = 1; t < M; ++t)
+      if (E[r * M + t] < E[r * M + tb]) tb = t;
+    std::memcpy(out + r * n, &sp[(size_t)((r * M + tb) * n)], n);
+  }
+  return OJMI_OK;
+}
+
+int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
+  if (!c || !info) return fail(OJMI_EINVAL, "null argument");
+  info->num_spins = c->n;
+  info->num_replicas = c->R;
+  info->num_trotter = c->M;
+  info->precision = c->prec;
+  info->kernel = c->family;
+  info->refresh_every = c->refresh_every;
+  info->coupling_bytes = (c->jtype == JT_F64) ? 8 : (c->jtype == JT_I8 ? 1 : 4);
+  info->last_sweeps = c->last_T;
+  info->last_attempts = c->last_attempts;
+  info->last_kernel_ms = c->last_ms;
+  info->last_refresh_ms = c->last_refresh_ms;
+  return OJMI_OK;
+}
+
+int ojmi_destroy(ojmi_ctx* c) {
+  if (!c) return OJMI_OK;
+  (void)hipStreamSynchronize(c->st);
+  delete c;
+  return OJMI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,59 @@
+// ojmi_kernels.h -- argument blocks and launchers shared between the C ABI
+// (ojmi_api.hip) and the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ojmi {
+
+enum FieldType { FT_F64 = 0, FT_F32 = 1, FT_I32 = 2 };
+enum StoreType { JT_F64 = 0, JT_F32 = 1, JT_I8 = 2, JT_I32 = 3 };
+
+// Per-sweep parameters (one entry per expanded sweep t):
+//   classical: p0 = -1/beta
+//   transverse: p0 = s*beta/M, p1 = 1/2 log tanh(beta*Gamma*(1-s)/M)
+struct DenseArgs {
+  const void* J;       // device coupling rows (layout in ojmi_wave.h)
+  const void* h;       // Npad fields' constant term (h_i), field type
+  int8_t* spins;       // [R][M][n] +-1
+  int n, R, M, T;
+  uint64_t seed, t0;   // Philox key, first global sweep index
+  int rep_offset;      // replica index offset of replica 0 in the stream
+  const double* p0_d;
+  const float* p0_f;
+  const double* p1_d;
+  const float* p1_f;
+  const void* F_in;    // optional [R][Npad] initial fields (field type)
+  void* F_out;         // optional [R][Npad] final fields
+};
+
+// CSR in level order (k_csr.hip): spins sorted by level, neighbours in ELL
+// form [D][n] indexed by level-order position.
+struct CsrArgs {
+  const int32_t* order;    // [n] spin index at level-order position
+  const int32_t* lvl_ptr;  // [nlev+1]
+  int nlev;
+  const int32_t* deg;      // [n] degree at position
+  const int32_t* ell_col;  // [D][n]
+  const void* val;         // [D][n], field type
+  int D;
+  const void* h;           // [n] at position, field type
+  int8_t* spins;           // [R][n]
+  int n, R, M, T;
+  uint64_t seed, t0;
+  int rep_offset;
+  const double* p0_d;
+  const float* p0_f;
+};
+
+hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
+hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, hipStream_t st);
+
+// FP64 fields/energies: H[c][j] = sum_i S[c][i] J64[i][j] (c over C configurations),
+// E[c] = sum_j s_j (H_j/2 + h_j); optional drift = max |F32[c][j] - (H + h)|.
+hipError_t launch_energy_f64(const double* J64, const double* h64, const int8_t* spins, int n,
+                             int Npad, int C, double* H, double* E, const float* F32,
+                             double* drift, hipStream_t st);
+
+}  // namespace ojmi

@@ -1,0 +1,127 @@
+// ojmi_wave.h -- register-resident local fields for the wave-per-replica
+// dense kernels.
+//
+// Layout: a replica (or a Trotter slice of one) is owned by one wave64.  Lane l
+// holds the local fields F_j of the spins j = 64*m + l for slots m = 0..NS-1,
+// and the spin signs as bit m of a 64-bit word (bit set = spin -1).  The
+// fields are kept in clang vectors of at most 1024 bits, which the compiler
+// keeps in VGPRs; a wave-uniform dynamic slot index (the slot being decided)
+// becomes s_set_gpr_idx indirect addressing instead of scratch memory.
+//
+// Device coupling layout (row i of the padded Npad x Npad matrix, Npad = 64*NS):
+//   fp64 / fp32 / int32 : element j at position j (natural order)
+//   int8               : element (m, l) at byte ((m>>2)*64 + l)*4 + (m&3), so
+//                        that one dword load per lane fetches 4 slots and one
+//                        wave instruction reads 256 contiguous bytes.
+#pragma once
+#include "ojmi_device.h"
+
+namespace ojmi {
+
+template <typename FT, int NS>
+struct FieldVec {
+  static constexpr int EPV = (sizeof(FT) == 8) ? 16 : 32;
+  static constexpr int VL0 = NS < EPV ? NS : EPV;
+  static constexpr int VL = VL0 < 2 ? 2 : VL0;
+  static constexpr int NV = (NS + VL - 1) / VL;
+  typedef FT vec_t __attribute__((ext_vector_type(VL)));
+  vec_t v[NV];
+
+  __device__ __forceinline__ FT get(int m) const { return v[m / VL][m % VL]; }
+  __device__ __forceinline__ void set(int m, FT x) { v[m / VL][m % VL] = x; }
+  // k is wave-uniform
+  __device__ __forceinline__ FT dyn(int k) const {
+    if constexpr (NV == 1) {
+      return v[0][k];
+    } else {
+      const int q = k / VL, e = k - q * VL;
+      FT r = v[0][e];
+#pragma unroll
+      for (int qq = 1; qq < NV; ++qq)
+        if (q == qq) r = v[qq][e];
+      return r;
+    }
+  }
+};
+
+// Number of bytes between consecutive rows of the device coupling matrix.
+template <typename JT, int NS>
+__host__ __device__ constexpr int64_t row_bytes() {
+  if constexpr (sizeof(JT) == 1) {
+    return (int64_t)((NS + 3) / 4) * 256;
+  } else {
+    return (int64_t)NS * 64 * (int64_t)sizeof(JT);
+  }
+}
+
+// Value of element (slot m, lane) of a coupling row.
+template <typename JT, typename FT, int NS>
+struct RowLoad {
+  FT x[NS];
+  __device__ __forceinline__ void load(const char* __restrict__ row, int lane) {
+    if constexpr (sizeof(JT) == 1) {
+      constexpr int NQ = (NS + 3) / 4;
+      uint32_t w[NQ];
+      const uint32_t* p = reinterpret_cast<const uint32_t*>(row);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) w[q] = p[q * 64 + lane];
+#pragma unroll
+      for (int m = 0; m < NS; ++m) {
+        const int32_t b = ((int32_t)(w[m >> 2] << (24 - 8 * (m & 3)))) >> 24;
+        x[m] = (FT)b;
+      }
+    } else {
+      const JT* p = reinterpret_cast<const JT*>(row);
+#pragma unroll
+      for (int m = 0; m < NS; ++m) x[m] = (FT)p[m * 64 + lane];
+    }
+  }
+};
+
+template <typename FT>
+__device__ __forceinline__ FT fma_ft(FT a, FT b, FT c);
+template <>
+__device__ __forceinline__ double fma_ft<double>(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+template <>
+__device__ __forceinline__ float fma_ft<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+template <>
+__device__ __forceinline__ int32_t fma_ft<int32_t>(int32_t a, int32_t b, int32_t c) {
+  return a * b + c;
+}
+
+// F += c * row  (c = +-2, so c * J is exact and each field sees one rounding)
+template <typename JT, typename FT, int NS>
+__device__ __forceinline__ void row_axpy(FieldVec<FT, NS>& F, const char* __restrict__ row,
+                                         int lane, FT c) {
+  RowLoad<JT, FT, NS> r;
+  r.load(row, lane);
+#pragma unroll
+  for (int m = 0; m < NS; ++m) F.set(m, fma_ft<FT>(c, r.x[m], F.get(m)));
+}
+
+// F = h + sum_k s_k J[k][.], k = 0..n-1 in order (one rounding per term).
+// `down` holds the lane's spin bits; the wave-uniform spin of k comes from a
+// ballot of slot k>>6.
+template <typename JT, typename FT, int NS>
+__device__ __forceinline__ void init_fields_seq(FieldVec<FT, NS>& F, const char* __restrict__ J,
+                                                const FT* __restrict__ h, uint64_t down,
+                                                int n, int lane) {
+#pragma unroll
+  for (int m = 0; m < NS; ++m) F.set(m, h[m * 64 + lane]);
+  constexpr int64_t RB = row_bytes<JT, NS>();
+  for (int q = 0; q < NS; ++q) {
+    const uint64_t bal = ballot64((down >> q) & 1ull);
+    const int kend = min(64, n - 64 * q);
+    for (int l2 = 0; l2 < kend; ++l2) {
+      const int k = 64 * q + l2;
+      const FT c = ((bal >> l2) & 1ull) ? (FT)(-1) : (FT)1;
+      row_axpy<JT, FT, NS>(F, J + (int64_t)k * RB, lane, c);
+    }
+  }
+}
+
+}  // namespace ojmi

@@ -1,0 +1,230 @@
+"""Simulated annealing sampler on the GPU engine.
+
+Same call surface and defaults as the reference SASampler.sample
+(openjij/sampler/sa_sampler.py:137-329): num_sweeps 1000, num_reads 1,
+geometric beta schedule with automatic bounds, ``sparse=True`` by default.
+The per-read loop of the reference (openjij/sampler/sampler.py:115-140) is
+replaced by one batched launch over all ``num_reads`` replicas.
+"""
+from __future__ import annotations
+
+import time
+from typing import Optional
+
+import numpy as np
+
+from openjij_amd._native import cxxjij
+from openjij_amd.model.model import BinaryQuadraticModel
+from openjij_amd.sampler.response import Response
+from openjij_amd.sampler.sampler import (
+    BaseSampler,
+    initial_states,
+    parse_initial_state,
+    resolve_seed,
+    to_vartype_states,
+)
+
+_PRECISIONS = {
+    None: 0, "auto": 0, "fp64": 1, "float64": 1, "fp32": 2, "mixed": 2, "float32": 2,
+    "int32": 3, "exact": 3,
+}
+
+
+def _precision_code(p) -> int:
+    if isinstance(p, int):
+        return p
+    key = p.lower() if isinstance(p, str) else p
+    if key not in _PRECISIONS:
+        raise ValueError(f"unknown precision {p!r}")
+    return _PRECISIONS[key]
+
+
+def geometric_ising_beta_schedule(cxxgraph, beta_max=None, beta_min=None, num_sweeps=1000,
+                                  seed=None):
+    """Geometric beta schedule with automatic bounds, as the reference builds it
+    (openjij/sampler/sa_sampler.py:853-976): the bounds come from local-field
+    statistics of two random spin vectors drawn with NumPy's legacy global
+    stream after ``np.random.seed(seed)``; one_mc_step = max(1, num_sweeps // 1000).
+    Returns (schedule list, [beta_max, beta_min])."""
+    if seed is not None:
+        np.random.seed(seed)
+    tiny = 1e-8
+    dE_max, dE_min, rate = 1.0, 1.0, 0.5
+    mat = cxxgraph.get_interactions()
+    dense = isinstance(mat, np.ndarray)
+    n1 = mat.shape[0]
+    if beta_min is None or beta_max is None:
+        if n1 <= 1:
+            dE_max = dE_min = 1
+        else:
+            spins = np.random.choice([-1, 1], size=(n1, 2))
+            spins[-1, :] = 1
+            quad = mat[:-1, :-1] @ spins[:-1, :] * (-2 * spins[:-1, :])
+            lin_col = mat[:-1, -1][:, np.newaxis] if dense else mat[:-1, -1].toarray()
+            lin = lin_col * spins[:-1]
+            quad_abs = np.abs(quad)
+            big = quad_abs > tiny
+            if np.any(big):
+                rate = np.max(np.abs(lin[big]) / (quad_abs[big].mean() + tiny))
+            pos = quad[quad > tiny]
+            if len(pos) == 0:
+                dE_min, dE_max = 1, 1.1
+            else:
+                dE_max = np.median(pos, axis=0).mean()
+                dE_min = np.min(pos, axis=0).mean()
+    n = n1
+    p_init = 9 / 10 - 8 / 10 * np.tanh(rate / 50) + tiny
+    p_final = 1 / 1000
+    if beta_min is None:
+        lo, hi = -np.log(p_init) / dE_max, -np.log(p_init) / dE_min
+        for _ in range(5):
+            mid = (lo + hi) / 2
+            f = 2 * mid * p_init - np.exp(-mid * dE_min) + np.exp(-mid * dE_max)
+            if abs(f) < 1e-8:
+                break
+            if f > 0:
+                hi = mid
+            else:
+                lo = mid
+        beta_min = lo
+        bmm = beta_min * 0.8
+        beta_min = bmm + (beta_min * 1.5 - bmm) * np.tanh(num_sweeps / n)
+    if beta_max is None:
+        absm = np.abs(mat)
+        if not dense:
+            absm = absm.toarray()
+        int_min = np.min(absm[absm > tiny])
+        int_min = max(int_min, dE_min / 10.0)
+        b_int = -np.log(p_final) / int_min
+        b_dE = -np.log(p_final) / dE_min
+        if rate < 2:
+            lo_, hi_ = min(b_dE, b_int), max(b_dE, b_int)
+            beta_max = lo_ + (hi_ - lo_) * np.tanh(num_sweeps / (2 * n))
+        else:
+            last = min(100, num_sweeps // 10 + 1)
+            beta_max = -np.log(p_final) / max(dE_min, int_min) / last
+        if beta_max < beta_min:
+            beta_max = beta_min * 10.0
+    if rate > 1:
+        beta_min *= rate
+        beta_max *= rate
+    per_beta = max(1, num_sweeps // 1000)
+    schedule = cxxjij.utility.make_classical_schedule_list(
+        beta_min=beta_min, beta_max=beta_max, one_mc_step=per_beta,
+        num_call_updater=num_sweeps // per_beta)
+    return schedule, [beta_max, beta_min]
+
+
+def schedule_arrays(schedule):
+    beta = np.array([e.updater_parameter.beta for e in schedule], dtype=float)
+    mcs = np.array([e.one_mc_step for e in schedule], dtype=np.int64)
+    return beta, mcs
+
+
+class SASampler(BaseSampler):
+    """Simulated annealing over classical Ising / QUBO models, batched on the
+    GPU: all ``num_reads`` replicas anneal in one kernel launch."""
+
+    def __init__(self):
+        self._default_params = {
+            "beta_min": None, "beta_max": None, "num_sweeps": 1000, "schedule": None,
+            "num_reads": 1,
+        }
+        self._params = dict(self._default_params)
+        self.schedule_info = {}
+
+    def _convert_validation_schedule(self, schedule):
+        """openjij/sampler/sa_sampler.py:98-135"""
+        if not isinstance(schedule, (list, np.ndarray)):
+            raise ValueError("schedule should be list or numpy.array")
+        if isinstance(schedule[0], cxxjij.utility.ClassicalSchedule):
+            return list(schedule)
+        if len(schedule[0]) != 2:
+            raise ValueError(
+                "schedule is list of tuple or list (beta : float, step_length : int)")
+        beta = np.array(schedule, dtype=float).T[0]
+        if not np.all(0 <= beta):
+            raise ValueError("schedule beta range is '0 <= beta'.")
+        out = []
+        for b, steps in schedule:
+            e = cxxjij.utility.ClassicalSchedule()
+            e.one_mc_step = int(steps)
+            e.updater_parameter.beta = float(b)
+            out.append(e)
+        return out
+
+    def sample(self, bqm, beta_min: Optional[float] = None, beta_max: Optional[float] = None,
+               num_sweeps: Optional[int] = None, num_reads: Optional[int] = None,
+               schedule: Optional[list] = None, initial_state=None,
+               updater: Optional[str] = None, sparse: Optional[bool] = None,
+               reinitialize_state: Optional[bool] = None, seed: Optional[int] = None,
+               precision=None) -> Response:
+        if updater is None:
+            updater = "single spin flip"
+        if sparse is None:
+            sparse = True
+        if reinitialize_state is None:
+            reinitialize_state = True
+        name = updater.lower().replace("_", "").replace(" ", "")
+        if name != "singlespinflip":
+            raise ValueError('updater is one of "single spin flip"')
+        if bqm.sparse != bool(sparse):
+            bqm = BinaryQuadraticModel(bqm.linear, bqm.quadratic, bqm.offset, bqm.vartype,
+                                       sparse=bool(sparse))
+        model = bqm
+        graph, offset = model.get_cxxjij_ising_graph()
+        self._set_params(beta_min=beta_min, beta_max=beta_max, num_sweeps=num_sweeps,
+                         num_reads=num_reads, schedule=schedule)
+        if self._params["schedule"] is None:
+            sched, rng = geometric_ising_beta_schedule(
+                graph, beta_max=self._params["beta_max"], beta_min=self._params["beta_min"],
+                num_sweeps=self._params["num_sweeps"], seed=seed)
+            self.schedule_info = {"beta_max": rng[0], "beta_min": rng[1],
+                                  "num_sweeps": self._params["num_sweeps"]}
+        else:
+            sched = self._convert_validation_schedule(self._params["schedule"])
+            self.schedule_info = {"schedule": "custom schedule"}
+        beta, mcs = schedule_arrays(sched)
+        R = int(self._params["num_reads"])
+        n = graph.size()
+        seed_v = resolve_seed(seed)
+        if initial_state is None:
+            init = initial_states(R, n, seed_v, 0x5A)
+        else:
+            init = np.tile(parse_initial_state(initial_state, model, n), (R, 1))
+
+        t0 = time.perf_counter()
+        info = {}
+        if n == 0:
+            spins = np.zeros((R, 0), dtype=np.int8)
+            exec_us = np.zeros(R)
+        elif reinitialize_state:
+            eng = cxxjij.engine.Engine(graph, R, 1, 0.0, _precision_code(precision))
+            eng.set_spins(init)
+            eng.run(beta, None, mcs, seed_v, 0)
+            spins = eng.solution()
+            inf = eng.info()
+            exec_us = np.full(R, inf["last_kernel_ms"] * 1e3 / R)
+            info["engine"] = inf
+        else:
+            # reads continue from the previous read's final state
+            eng = cxxjij.engine.Engine(graph, 1, 1, 0.0, _precision_code(precision))
+            cur = init[:1]
+            spins = np.empty((R, n), dtype=np.int8)
+            exec_us = np.empty(R)
+            sweeps = int(mcs.sum())
+            for r in range(R):
+                eng.set_spins(cur)
+                eng.run(beta, None, mcs, seed_v, r * sweeps)
+                cur = eng.spins()
+                spins[r] = cur[0]
+                exec_us[r] = eng.info()["last_kernel_ms"] * 1e3
+        states = to_vartype_states(spins, model.vartype)
+        energies = model.energies(states) if n else np.full(R, model.offset)
+        sampling_us = (time.perf_counter() - t0) * 1e6
+        resp = Response(states, energies, model.variables, model.vartype, info)
+        resp.info["sampling_time"] = sampling_us
+        resp.info["execution_time"] = float(np.mean(exec_us))
+        resp.info["list_exec_times"] = exec_us
+        resp.info["schedule"] = self.schedule_info
+        return resp

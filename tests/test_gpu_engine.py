@@ -1,0 +1,247 @@
+"""GPU parity of the HIP kernels through the C ABI, against the CPU oracle.
+
+Bit-exact: the Philox replay reproduces the GPU spin trajectories for integer
+couplings on every path, and for real couplings on the fp64 and fp32 paths
+(fields are accumulated in the same order on both sides).  Energies are
+compared with the exact FP64 evaluation (equal for integer couplings,
+relative 1e-12 for real ones).  Ground states are checked against the
+reference fixtures and brute force; distributions against the reference
+restatement with a two-sample KS test (threshold p > 1e-3).
+"""
+import numpy as np
+import pytest
+
+from openjij_amd import capi
+from oracle import oracle as orc
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+PREC = {"fp64": capi.PREC_FP64, "fp32": capi.PREC_FP32, "int32": capi.PREC_INT32}
+
+
+def sched(L, bmin=0.1, bmax=5.0, mcs=1):
+    return H.geometric_betas(bmin, bmax, L), np.full(L, mcs, dtype=np.int64)
+
+
+def run_dense(E_upper, prec, R, beta, mcs, seed, offset=0, init_seed=1):
+    n = E_upper.shape[0] - 1
+    init = H.random_pm1(R, n, init_seed)
+    with capi.Context.dense(E_upper, R, 1, 0.0, PREC[prec]) as ctx:
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, seed, offset)
+        got = ctx.get_spins()
+        info = ctx.info()
+        energies = ctx.energies()
+    want = orc.replay_dense(prec, H.symmetric_ext(E_upper), init, seed, offset,
+                            orc.expand_classical(beta, mcs))
+    return init, got, want, info, energies
+
+
+@pytest.mark.parametrize("n", [5, 63, 64, 65, 130, 256, 700])
+@pytest.mark.parametrize("prec", ["fp64", "int32"])
+def test_dense_integer_couplings_bitexact(n, prec):
+    E = H.int_dense(n, 1000 + n)
+    beta, mcs = sched(40, 0.05, 2.0)
+    init, got, want, info, en = run_dense(E, prec, 6, beta, mcs, seed=12345 + n)
+    assert info["kernel"] == capi.KERNEL_DENSE_WAVE
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(en, orc.energy_exact(E, got))
+
+
+@pytest.mark.parametrize("n", [17, 64, 200, 1024])
+def test_dense_real_couplings_fp64_bitexact(n):
+    E = H.sk_gauss(n, 2000 + n, field=True)
+    beta, mcs = sched(30, 0.1, 8.0)
+    _, got, want, info, en = run_dense(E, "fp64", 4, beta, mcs, seed=99)
+    np.testing.assert_array_equal(got, want)
+    ex = orc.energy_exact(E, got)
+    assert np.all(np.abs(en - ex) <= 1e-12 * (1 + np.abs(ex)))
+
+
+@pytest.mark.parametrize("n", [33, 256, 2048])
+def test_dense_fp32_bitexact_without_refresh(n):
+    E = H.sk_gauss(n, 3000 + n, field=True)
+    beta, mcs = sched(20, 0.1, 8.0)
+    _, got, want, info, _ = run_dense(E, "fp32", 3, beta, mcs, seed=7)
+    assert info["coupling_bytes"] == 4
+    np.testing.assert_array_equal(got, want)
+
+
+def test_dense_fp64_and_int32_paths_agree():
+    E = H.sk_pm1(300, 5)
+    beta, mcs = sched(25, 0.1, 3.0)
+    _, a, _, _, _ = run_dense(E, "fp64", 8, beta, mcs, seed=5)
+    _, b, _, info, _ = run_dense(E, "int32", 8, beta, mcs, seed=5)
+    assert info["coupling_bytes"] == 1
+    np.testing.assert_array_equal(a, b)
+
+
+def test_split_run_equals_single_run():
+    E = H.int_dense(150, 77)
+    beta, mcs = sched(30)
+    init = H.random_pm1(4, 150, 3)
+    with capi.Context.dense(E, 4) as c1, capi.Context.dense(E, 4) as c2:
+        c1.set_spins(init)
+        c1.run(beta, mcs, 42, 0)
+        c2.set_spins(init)
+        c2.run(beta[:13], mcs[:13], 42, 0)
+        c2.run(beta[13:], mcs[13:], 42, 13)
+        np.testing.assert_array_equal(c1.get_spins(), c2.get_spins())
+
+
+def test_config_a_sk256_pm1_replay():
+    """Dense SK N=256, J = +-1, 64 reads, 1000 sweeps, beta 0.1 -> 10."""
+    E = H.sk_pm1(256, 0xA001)
+    beta, mcs = sched(1000, 0.1, 10.0)
+    for prec in ("fp64", "int32"):
+        _, got, want, info, en = run_dense(E, prec, 64, beta, mcs, seed=0xA001)
+        np.testing.assert_array_equal(got, want)
+        assert info["last_attempts"] == 64 * 1000 * 256
+        np.testing.assert_array_equal(en, orc.energy_exact(E, got))
+
+
+# --- CSR --------------------------------------------------------------------
+@pytest.mark.parametrize("L,R,sweeps", [(3, 4, 30), (6, 16, 20), (16, 4, 6)])
+def test_csr_ea3d_int_bitexact(L, R, sweeps):
+    n, rp, ci, v = H.ea3d_csr(L, 0xB002 + L)
+    beta, mcs = sched(sweeps, 0.1, 5.0)
+    init = H.random_pm1(R, n, L)
+    with capi.Context.csr(n, rp, ci, v, R, precision=capi.PREC_INT32) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_CSR_LEVEL
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 31337, 0)
+        got = ctx.get_spins()
+        en = ctx.energies()
+    want = orc.replay_csr("int32", rp, ci, v, n, init, 31337, 0, orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(got, want)
+    for r in range(R):
+        assert en[r] == orc.energy_exact_csr(rp, ci, v, n, got[r])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_csr_real_couplings_bitexact(prec):
+    E = H.sk_gauss(90, 4242, field=True)
+    # sparsify: keep ~8 neighbours per spin
+    mask = (H.splitmix64_np(5, 91 * 91).reshape(91, 91) >> np.uint64(60)) == 0
+    E = np.where(mask | (np.arange(91)[None, :] == 90), E, 0.0)
+    S = H.symmetric_ext(E)
+    rp, ci, v = orc.dense_to_csr(S)
+    beta, mcs = sched(25, 0.1, 6.0)
+    init = H.random_pm1(5, 90, 8)
+    with capi.Context.csr(90, rp, ci, v, 5, precision=PREC[prec]) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_CSR_LEVEL
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 11, 0)
+        got = ctx.get_spins()
+    want = orc.replay_csr(prec, rp, ci, v, 90, init, 11, 0, orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(got, want)
+
+
+def test_csr_dense_like_graph_runs_on_dense_kernel():
+    E = H.symmetric_ext(H.int_dense(200, 6, density=0.9))
+    rp, ci, v = orc.dense_to_csr(E)
+    beta, mcs = sched(10)
+    init = H.random_pm1(3, 200, 4)
+    with capi.Context.csr(200, rp, ci, v, 3) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_DENSE_WAVE
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 2, 0)
+        got = ctx.get_spins()
+    want = orc.replay_dense("int32", E, init, 2, 0, orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(got, want)
+
+
+# --- Trotter ------------------------------------------------------------------
+@pytest.mark.parametrize("n,M,prec", [(32, 2, "fp64"), (32, 3, "fp64"), (40, 4, "fp32"),
+                                      (70, 5, "fp64"), (64, 8, "fp32"), (130, 33, "fp32")])
+def test_sqa_dense_bitexact(n, M, prec):
+    E = H.sk_gauss(n, 5000 + n + M, field=True)
+    L = 20
+    s = np.linspace(0.0, 1.0, L)
+    beta = np.full(L, 4.0)
+    mcs = np.ones(L, dtype=np.int64)
+    gamma = 2.0
+    R = 3
+    init = np.repeat(H.random_pm1(R, n, M)[:, None, :], M, axis=1)
+    with capi.Context.dense(E, R, M, gamma, PREC[prec]) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_SQA_WAVE
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 1234, 0, s=s)
+        got = ctx.get_spins()
+        sol = ctx.solution()
+        en = ctx.energies()
+    a, B = orc.expand_transverse(beta, s, mcs, gamma, M)
+    want = orc.replay_sqa_dense(prec, H.symmetric_ext(E), init, 1234, 0, a, B)
+    np.testing.assert_array_equal(got, want)
+    for r in range(R):
+        ex = orc.energy_exact(E, got[r])
+        assert np.all(np.abs(en[r] - ex) <= 1e-12 * (1 + np.abs(ex)))
+        assert list(sol[r]) == list(got[r, int(np.argmin(en[r]))])
+
+
+def test_sqa_config_c_shape_replay():
+    """SK N=256 Gaussian, M=32, Gamma=5, s linear to 4.99/5 (a short slice of
+    the 1000-sweep schedule), fp32 and fp64."""
+    n, M, R = 256, 32, 2
+    E = H.sk_gauss(n, 0xC003)
+    L = 8
+    s = (4.99 / 5) * np.arange(L) / 999.0
+    beta = np.full(L, 10.0)
+    mcs = np.ones(L, dtype=np.int64)
+    init = np.repeat(H.random_pm1(R, n, 3)[:, None, :], M, axis=1)
+    a, B = orc.expand_transverse(beta, s, mcs, 5.0, M)
+    for prec in ("fp32", "fp64"):
+        with capi.Context.dense(E, R, M, 5.0, PREC[prec]) as ctx:
+            ctx.set_spins(init)
+            ctx.run(beta, mcs, 0xC003, 0, s=s)
+            got = ctx.get_spins()
+        want = orc.replay_sqa_dense(prec, H.symmetric_ext(E), init, 0xC003, 0, a, B)
+        np.testing.assert_array_equal(got, want)
+
+
+# --- ground states and statistics ---------------------------------------------
+def test_reference_fixture_ground_states(fixtures):
+    c = fixtures["classical_schedule"]
+    beta = H.geometric_betas(c["beta_min"], c["beta_max"], c["num_call_updater"])
+    mcs = np.full(len(beta), c["one_mc_step"], dtype=np.int64)
+    for k, case in fixtures["testcases"].items():
+        E = np.array(case["J_ext_upper"], dtype=float)
+        with capi.Context.dense(E, 32) as ctx:
+            ctx.set_spins(H.random_pm1(32, 8, int(k)))
+            ctx.run(beta, mcs, 5678, 0)
+            got = ctx.get_spins()
+        hits = sum(list(s) == case["ground_state"] for s in got)
+        assert hits >= (8 if case["hard"] else 28), (k, hits)
+
+
+@pytest.mark.parametrize("n", [12, 18, 22])
+def test_brute_force_ground_state_found(n):
+    E = H.sk_gauss(n, 7000 + n, field=True)
+    e0, s0 = orc.brute_force(E)
+    beta, mcs = sched(300, 0.1, 10.0)
+    with capi.Context.dense(E, 64) as ctx:
+        ctx.set_spins(H.random_pm1(64, n, 2))
+        ctx.run(beta, mcs, 3, 0)
+        en = ctx.energies()
+    assert en.min() == pytest.approx(e0, abs=1e-9)
+
+
+def test_energy_distribution_matches_reference_ks():
+    """Two-sample KS between the GPU sampler and the reference restatement
+    (Xorshift + libstdc++ uniforms) at the same schedule: SK N=64 +-1, 1000
+    reads each, 60 sweeps beta 0.1 -> 2.  Threshold p > 1e-3."""
+    n, R = 64, 1000
+    E = H.sk_pm1(n, 0x5EED)
+    beta, mcs = sched(60, 0.1, 2.0)
+    init = H.random_pm1(R, n, 77)
+    with capi.Context.dense(E, R, precision=capi.PREC_FP64) as ctx:
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 2024, 0)
+        e_gpu = ctx.energies()
+    ref = orc.ref_sa_dense_batch(H.symmetric_ext(E), init, np.arange(R, dtype=np.uint32) + 1,
+                                 beta, mcs)
+    e_ref = orc.energy_exact(E, ref)
+    p = H.ks_2samp_pvalue(e_gpu, e_ref)
+    print(f"KS p-value GPU vs reference restatement: {p:.4f}")
+    assert p > 1e-3

@@ -1,0 +1,298 @@
+"""Pins of the CPU oracle (oracle/oj_oracle.cpp) against known answers, the
+reference's own fixtures, closed forms and brute force.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from tests import helpers as H
+
+
+# --- random streams ----------------------------------------------------------
+def test_philox_random123_known_answers():
+    # Random123 kat_vectors for philox4x32_10
+    assert [hex(x) for x in orc.philox([0, 0, 0, 0], [0, 0])] == [
+        "0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    assert [hex(x) for x in orc.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)] == [
+        "0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(x) for x in orc.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                                       [0xA4093822, 0x299F31D0])] == [
+        "0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+
+
+def test_philox_word_counter_layout():
+    # spin i uses counter (i>>2, t, r, m), word i&3
+    seed = 0x0123456789ABCDEF
+    for (t, r, m, i) in [(0, 0, 0, 0), (7, 3, 1, 13), (999, 2047, 31, 4095)]:
+        full = orc.philox([i >> 2, t, r, m], [seed & 0xFFFFFFFF, seed >> 32])
+        assert orc.philox_word(seed, t, r, m, i) == full[i & 3]
+
+
+def test_xorshift_marsaglia_first_output():
+    # Marsaglia (2003) xor128: with w = 88675123 the first output is 3701687786
+    assert orc.ref_xorshift(88675123, 1)[0] == 3701687786
+
+
+def test_uniform_uses_two_draws_per_value():
+    # libstdc++ generate_canonical<double,53> over a 2^32-range engine
+    x = orc.ref_xorshift(1234, 20).astype(np.float64)
+    u = orc.ref_uniform(1234, 10)
+    expect = (x[0::2] + x[1::2] * 2.0 ** 32) / 2.0 ** 64
+    np.testing.assert_array_equal(u, np.minimum(expect, np.nextafter(1.0, 0.0)))
+
+
+def test_det_log_accuracy_and_edges():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.random(20000), [0.5, 0.25, 1 - 2 ** -33, 2 ** -33, 0.70710678]])
+    xs = xs[(xs > 0) & (xs < 1)]
+    for x in xs[:5000]:
+        d = orc.det_log_d(x)
+        assert abs(d - math.log(x)) <= 4e-16 * abs(math.log(x)) + 1e-300
+    for x in xs[:5000].astype(np.float32):
+        f = orc.det_log_f(x)
+        assert abs(f - math.log(float(x))) <= 3e-7 * abs(math.log(float(x)))
+    assert orc.det_log_d(0.5) == -math.log(2.0)
+    # log(U) < 0 for the largest uniform of each width (so dE <= 0 always flips)
+    assert orc.det_log_d((2 ** 32 - 1 + 0.5) * 2.0 ** -32) < 0
+    assert orc.det_log_f(((2 ** 23 - 1) + 0.5) * 2.0 ** -23) < 0
+
+
+def test_det_log_monotone():
+    xs = np.sort(np.random.default_rng(1).random(3000))
+    v = [orc.det_log_d(x) for x in xs]
+    assert all(b >= a for a, b in zip(v, v[1:]))
+
+
+# --- schedules -----------------------------------------------------------------
+def test_classical_schedule_closed_form(fixtures):
+    c = fixtures["classical_schedule"]
+    b = orc.classical_schedule(c["beta_min"], c["beta_max"], c["num_call_updater"])
+    assert b[0] == c["beta_min"]
+    assert abs(b[-1] - c["beta_max"]) < 1e-10 * c["beta_max"]
+    r = b[1:] / b[:-1]
+    assert np.allclose(r, (c["beta_max"] / c["beta_min"]) ** (1 / (c["num_call_updater"] - 1)))
+    # same iterated multiplication as the test helper
+    np.testing.assert_array_equal(b, H.geometric_betas(c["beta_min"], c["beta_max"],
+                                                       c["num_call_updater"]))
+
+
+def test_transverse_schedule_closed_form():
+    s = orc.transverse_schedule(100)
+    assert s[0] == 0.0
+    assert abs(s[-1] - 1.0) < 1e-12
+    assert np.allclose(np.diff(s), 1 / 99)
+
+
+# --- energies ------------------------------------------------------------------
+def test_energy_check_closed_forms(fixtures):
+    # tests/cxxtest/graph/quadratic.hpp:129-195
+    N = fixtures["energy_check"]["N"]
+    E = np.zeros((N + 1, N + 1))
+    E[:N, :N] = np.triu(np.ones((N, N)), 1)
+    E[N, N] = 1
+    up, dn = np.ones(N, np.int8), -np.ones(N, np.int8)
+    assert orc.energy_exact(E, up) == (N * N - N) / 2
+    assert orc.energy_exact(E, dn) == (N * N - N) / 2
+    E[:N, N] = 1.0
+    assert orc.energy_exact(E, up) == (N * N + N) / 2
+    assert orc.energy_exact(E, dn) == (N * N - 3 * N) / 2
+    rp, ci, v = orc.dense_to_csr(E)
+    assert orc.energy_exact_csr(rp, ci, v, N, up) == (N * N + N) / 2
+    assert orc.energy_exact_csr(rp, ci, v, N, dn) == (N * N - 3 * N) / 2
+
+
+def test_energy_matches_quadratic_form():
+    E = H.sk_gauss(40, 7, field=True)
+    S = H.random_pm1(5, 40, 3).astype(float)
+    Jsym = H.symmetric_ext(E)[:40, :40]
+    h = E[:40, 40]
+    for s in S:
+        expect = 0.5 * s @ Jsym @ s + h @ s
+        assert abs(orc.energy_exact(E, s.astype(np.int8)) - expect) < 1e-12 * (1 + abs(expect))
+
+
+# --- brute force ----------------------------------------------------------------
+def _enumerate(E):
+    n = E.shape[0] - 1
+    best, arg = None, None
+    for k in range(1 << n):
+        s = np.array([1 - 2 * ((k >> i) & 1) for i in range(n)], np.int8)
+        e = orc.energy_exact(E, s)
+        if best is None or e < best - 1e-12:
+            best, arg = e, s
+    return best, arg
+
+
+def test_brute_force_matches_enumeration():
+    for seed in range(4):
+        E = H.sk_gauss(10, 100 + seed, field=True)
+        e, s = orc.brute_force(E)
+        e2, s2 = _enumerate(E)
+        assert abs(e - e2) < 1e-12
+        assert abs(orc.energy_exact(E, s) - e2) < 1e-12
+
+
+def _case_ext(case):
+    return np.array(case["J_ext_upper"], dtype=float)
+
+
+def test_fixture_ground_states_are_brute_force_minima(fixtures):
+    expected = {"1": -21.5, "2": -25.9, "3": -26.6, "4": -22.1, "5": -27.33}
+    for k, case in fixtures["testcases"].items():
+        E = _case_ext(case)
+        e, s = orc.brute_force(E)
+        gs = np.array(case["ground_state"], np.int8)
+        assert orc.energy_exact(E, gs) == pytest.approx(e, abs=1e-12)
+        assert e == pytest.approx(expected[k], abs=1e-9)
+        assert list(s) == list(gs)
+
+
+# --- reference restatement ---------------------------------------------------------
+def test_ref_sa_finds_fixture_ground_states(fixtures):
+    """SingleSpinFlip.FindTrueGroundState_ClassicalIsing_Dense with the test
+    schedule (beta 0.1 -> 100, 100 levels x 100 sweeps)."""
+    c = fixtures["classical_schedule"]
+    beta = orc.classical_schedule(c["beta_min"], c["beta_max"], c["num_call_updater"])
+    mcs = np.full(len(beta), c["one_mc_step"])
+    for k, case in fixtures["testcases"].items():
+        E = H.symmetric_ext(_case_ext(case))
+        hits = 0
+        for seed in range(5):
+            init = orc.ref_gen_spin(8, 1234 + seed)
+            out = orc.ref_sa_dense(E, init, 5678 + seed, beta, mcs)
+            hits += list(out) == case["ground_state"]
+        # the "Hard Instance" (case 3, gap 0.2) is found less often
+        assert hits >= (2 if case["hard"] else 4), (k, hits)
+
+
+def test_ref_sa_csr_equals_dense_for_integer_couplings():
+    E = H.symmetric_ext(H.int_dense(24, 5))
+    rp, ci, v = orc.dense_to_csr(E)
+    beta = H.geometric_betas(0.1, 3.0, 50)
+    mcs = np.full(50, 2)
+    init = H.random_pm1(6, 24, 9)
+    seeds = np.arange(6, dtype=np.uint32) + 11
+    a = orc.ref_sa_dense_batch(E, init, seeds, beta, mcs)
+    b = orc.ref_sa_csr_batch(rp, ci, v, 24, init, seeds, beta, mcs)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_ref_sqa_finds_fixture_ground_state(fixtures):
+    t = fixtures["transverse_schedule"]
+    s = orc.transverse_schedule(t["num_call_updater"])
+    beta = np.full(len(s), t["beta"])
+    mcs = np.full(len(s), t["one_mc_step"])
+    case = fixtures["testcases"]["1"]
+    E = H.symmetric_ext(_case_ext(case))
+    M = t["num_trotter"]
+    hits = 0
+    for seed in range(3):
+        init = np.tile(orc.ref_gen_spin(8, 1234 + seed), (M, 1))
+        ts = orc.ref_sqa_dense(E, t["gamma"], init, 5678 + seed, beta, s, mcs)
+        hits += list(orc.ref_solution_transverse(E, ts)) == case["ground_state"]
+    assert hits >= 2
+
+
+def test_ref_solution_transverse_picks_min_slice(fixtures):
+    # tests/cxxtest/result/result.hpp:21-41
+    f = fixtures["trotter_solution"]
+    n = f["n"]
+    E = np.zeros((n + 1, n + 1))
+    for i, j, v in f["J"]:
+        E[i, j] = v
+    for i, v in f["h"].items():
+        E[int(i), n] = v
+    E[n, n] = 1
+    S = H.symmetric_ext(E)
+    ts = np.array([f["slice0"]] + [list(orc.ref_gen_spin(n, 1234 + k)) for k in range(3)],
+                  np.int8)
+    assert list(orc.ref_solution_transverse(S, ts)) == f["slice0"]
+
+
+# --- GPU-rule replay ------------------------------------------------------------------
+def _nb(beta, mcs):
+    return orc.expand_classical(beta, mcs)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "int32"])
+def test_replay_integer_modes_agree(mode):
+    """fp64 and int32 replay follow the same trajectory on integer couplings
+    (both keep exact fields and an fp64 threshold)."""
+    E = H.symmetric_ext(H.int_dense(40, 3))
+    init = H.random_pm1(8, 40, 4)
+    nb = _nb(H.geometric_betas(0.05, 2.0, 30), np.ones(30))
+    a = orc.replay_dense("fp64", E, init, 77, 0, nb)
+    b = orc.replay_dense(mode, E, init, 77, 0, nb)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_replay_dense_equals_csr_on_integer_couplings():
+    """Incremental fields (dense) and per-attempt recomputed fields (CSR)
+    give the same trajectory when arithmetic is exact."""
+    E = H.symmetric_ext(H.int_dense(50, 8, density=0.2))
+    rp, ci, v = orc.dense_to_csr(E)
+    init = H.random_pm1(5, 50, 2)
+    nb = _nb(H.geometric_betas(0.1, 3.0, 40), np.ones(40))
+    a = orc.replay_dense("int32", E, init, 5, 3, nb)
+    b = orc.replay_csr("int32", rp, ci, v, 50, init, 5, 3, nb)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_replay_split_run_equals_single_run():
+    E = H.symmetric_ext(H.sk_pm1(30, 1))
+    init = H.random_pm1(3, 30, 1)
+    nb = _nb(H.geometric_betas(0.1, 5.0, 20), np.ones(20))
+    one = orc.replay_dense("fp64", E, init, 9, 0, nb)
+    half = orc.replay_dense("fp64", E, init, 9, 0, nb[:11])
+    two = orc.replay_dense("fp64", E, half, 9, 11, nb[11:])
+    np.testing.assert_array_equal(one, two)
+
+
+def test_replay_reaches_brute_force_ground_state():
+    for seed in range(3):
+        E = H.symmetric_ext(H.sk_gauss(14, 200 + seed, field=True))
+        e0, _ = orc.brute_force(E)
+        nb = _nb(H.geometric_betas(0.1, 10.0, 200), np.ones(200))
+        for mode in ("fp64", "fp32"):
+            out = orc.replay_dense(mode, E, H.random_pm1(16, 14, seed), 3, 0, nb)
+            best = min(orc.energy_exact(E, s) for s in out)
+            assert best == pytest.approx(e0, abs=1e-9), (seed, mode)
+
+
+def test_replay_beta_zero_accepts_everything():
+    E = H.symmetric_ext(H.sk_pm1(20, 2))
+    init = H.random_pm1(2, 20, 5)
+    out = orc.replay_dense("fp64", E, init, 1, 0, np.array([-np.inf]))
+    np.testing.assert_array_equal(out, -init)
+
+
+def test_replay_metropolis_rule_matches_reference_statistics():
+    """The threshold rule with Philox has the reference Metropolis statistics:
+    two-sample KS on final energies of SK N=24 after a short anneal,
+    GPU rule (replay) vs the reference restatement.  Threshold p > 1e-3."""
+    n, R = 24, 600
+    E = H.symmetric_ext(H.sk_gauss(n, 31))
+    beta = H.geometric_betas(0.1, 1.5, 20)
+    mcs = np.ones(20, dtype=np.int64)
+    init = H.random_pm1(R, n, 12)
+    a = orc.replay_dense("fp64", E, init, 4242, 0, _nb(beta, mcs))
+    b = orc.ref_sa_dense_batch(E, init, np.arange(R, dtype=np.uint32) * 7 + 1, beta, mcs)
+    ea, eb = orc.energy_exact(E, a), orc.energy_exact(E, b)
+    assert H.ks_2samp_pvalue(ea, eb) > 1e-3
+
+
+def test_replay_sqa_modes_and_edges():
+    n, M = 16, 5
+    E = H.symmetric_ext(H.sk_gauss(n, 41))
+    s = np.linspace(0, 1, 30)
+    a, B = orc.expand_transverse(np.full(30, 5.0), s, np.ones(30, np.int64), 1.0, M)
+    assert np.isneginf(B[-1])  # s = 1: tanh(0) = 0
+    init = np.repeat(H.random_pm1(4, n, 3)[:, None, :], M, axis=1)
+    out = orc.replay_sqa_dense("fp64", E, init, 8, 0, a, B)
+    assert out.shape == init.shape
+    rp, ci, v = orc.dense_to_csr(E)
+    out2 = orc.replay_sqa_csr("fp64", rp, ci, v, n, init, 8, 0, a, B)
+    # real couplings: recompute vs incremental may differ in the last bit, but
+    # the final energies agree statistically; here just well-formed +-1
+    assert set(np.unique(out2)) <= {-1, 1}
