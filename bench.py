@@ -30,17 +30,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(cfg):
+def workload(cfg, sweeps=None, replicas=None):
     from tests import helpers as H
 
     if cfg == "E":
-        n, R, L = 4096, 2048, 500
+        n, R, L = 4096, replicas or 2048, sweeps or 500
         return dict(name="dense SK N=4096 gaussian fp32 J, 2048 replicas, 500 sweeps, "
                          "beta 0.1->10 geometric", n=n, R=R,
                     E=H.sk_gauss(n, 0xE005), beta=H.geometric_betas(0.1, 10.0, L),
                     mcs=np.ones(L, np.int64), fast="fp32", plain="fp64", tol_rel=1e-9)
     if cfg == "A":
-        n, R, L = 256, 64, 1000
+        n, R, L = 256, replicas or 64, sweeps or 1000
         return dict(name="dense SK N=256 +-1 J, 64 reads, 1000 sweeps, beta 0.1->10",
                     n=n, R=R, E=H.sk_pm1(n, 0xA001), beta=H.geometric_betas(0.1, 10.0, L),
                     mcs=np.ones(L, np.int64), fast="int32", plain="fp64", tol_rel=0.0)
@@ -53,12 +53,14 @@ def prec_code(p):
     return {"fp64": capi.PREC_FP64, "fp32": capi.PREC_FP32, "int32": capi.PREC_INT32}[p]
 
 
-def time_path(W, prec, steps, warmup, seed0):
+def time_path(W, prec, steps, warmup, seed0, options=None):
     """Returns per-step wall seconds, per-step kernel ms, energies of the last step."""
     from openjij_amd import capi
     from tests import helpers as H
 
     ctx = capi.Context.dense(W["E"], W["R"], 1, 0.0, prec_code(prec))
+    for k, v in (options or {}).items():
+        ctx.set_option(k, v)
     walls, kms = [], []
     energies = None
     for k in range(warmup + steps):
@@ -115,6 +117,12 @@ def main():
     ap.add_argument("--config", default="E")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--sweeps", type=int, default=None,
+                    help="shorter schedule (profiling runs only; not a bench figure)")
+    ap.add_argument("--replicas", type=int, default=None)
+    ap.add_argument("--group", type=int, default=None, help="replicas per workgroup")
+    ap.add_argument("--xcd-window", type=int, default=None)
+    ap.add_argument("--refresh-every", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; replicas would shard across GPUs "
@@ -126,11 +134,19 @@ def main():
 
     if capi.device_count() < 1:
         raise SystemExit("bench.py: no HIP device visible")
-    W = workload(args.config)
+    W = workload(args.config, args.sweeps, args.replicas)
     attempts = W["R"] * int(W["mcs"].sum()) * W["n"]
     log(f"workload: {W['name']} ({attempts:.3e} flip attempts per step)")
 
-    walls, kms, e_fast, drift, info = time_path(W, W["fast"], args.steps, args.warmup, 1000)
+    opts = {}
+    if args.group is not None:
+        opts["group"] = args.group
+    if args.xcd_window is not None:
+        opts["xcd_window"] = args.xcd_window
+    if args.refresh_every is not None:
+        opts["refresh_every"] = args.refresh_every
+    walls, kms, e_fast, drift, info = time_path(W, W["fast"], args.steps, args.warmup, 1000,
+                                                opts)
     step_s = float(walls.mean())
     value = attempts / step_s
     kernel_ms = float(kms.mean())

@@ -65,7 +65,12 @@ struct ojmi_ctx {
   int sqa_ppw = 1, sqa_w = 1;
   int csr_wpb = 1;
   int refresh_every = 0;
-  int group = 1;
+  int group = 1;         // 1: one wave per replica; 4/8/16: replicas per workgroup
+  int xcd_window = 2;    // shared kernel: XCD pacing window in blocks (0 = off)
+  int lds_budget = 65536;
+  void* dJnat = nullptr;   // natural-layout couplings when dJ is packed
+  uint32_t* dprog = nullptr;
+  int64_t prog_cap = 0;
 
   // kernel-layout couplings
   void* dJ = nullptr;
@@ -99,7 +104,7 @@ struct ojmi_ctx {
     if (dJ == dJ64) dJ = nullptr;
     if (dh == dh64) dh = nullptr;
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
-                    dH, dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f};
+                    dH, dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     if (e0) (void)hipEventDestroy(e0);
@@ -150,6 +155,11 @@ int resolve_precision(int32_t req, bool integer, double max_abs_field, int* out)
 }
 
 int common_init(ojmi_ctx* c) {
+  int dev = 0, lds = 65536;
+  OJMI_HIP(hipGetDevice(&dev));
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      lds > 0)
+    c->lds_budget = lds;
   OJMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   OJMI_HIP(hipEventCreate(&c->e0));
   OJMI_HIP(hipEventCreate(&c->e1));
@@ -229,6 +239,14 @@ int upload_dense(ojmi_ctx* c, const std::vector<double>& Jn /* n x n symmetric *
   if ((rc = dalloc(&dJ, J8.size())) != OJMI_OK) return rc;
   c->dJ = dJ;
   OJMI_HIP(hipMemcpy(dJ, J8.data(), J8.size(), hipMemcpyHostToDevice));
+  // natural row layout for the workgroup-shared kernel
+  std::vector<int8_t> Jn8((size_t)(Np * Np), 0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < n; ++j) Jn8[i * Np + j] = (int8_t)Jn[i * n + j];
+  int8_t* dn8;
+  if ((rc = dalloc(&dn8, Jn8.size())) != OJMI_OK) return rc;
+  c->dJnat = dn8;
+  OJMI_HIP(hipMemcpy(dn8, Jn8.data(), Jn8.size(), hipMemcpyHostToDevice));
   return OJMI_OK;
 }
 
@@ -353,6 +371,10 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
   if ((rc = setup_dense_family(c)) || (rc = common_init(c)) || (rc = upload_dense(c, Jn, h))) {
     delete c;
     return rc;
+  }
+  if (c->M == 1 && c->ftype == FT_F32 && c->R >= 8) {
+    c->group = 8;
+    c->family = OJMI_KERNEL_DENSE_SHARED;
   }
   *out = c;
   return OJMI_OK;
@@ -563,8 +585,19 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     return OJMI_OK;
   }
   if (k == "group") {
-    if (value < 1 || value > 16) return fail(OJMI_EINVAL, "group must be in 1..16");
+    if (!(value == 1 || value == 4 || value == 8 || value == 16))
+      return fail(OJMI_EINVAL, "group must be 1, 4, 8 or 16");
+    if (value > 1 && (c->M != 1 || c->ftype == FT_F64))
+      return fail(OJMI_EINVAL, "group > 1 needs a classical FP32 or INT32 system");
+    if (c->family != OJMI_KERNEL_DENSE_WAVE && c->family != OJMI_KERNEL_DENSE_SHARED)
+      return fail(OJMI_EINVAL, "group applies to dense classical systems");
     c->group = (int)value;
+    c->family = value > 1 ? OJMI_KERNEL_DENSE_SHARED : OJMI_KERNEL_DENSE_WAVE;
+    return OJMI_OK;
+  }
+  if (k == "xcd_window") {
+    if (value < 0 || value > 1000000) return fail(OJMI_EINVAL, "xcd_window must be >= 0");
+    c->xcd_window = (int)value;
     return OJMI_OK;
   }
   return fail(OJMI_EINVAL, "unknown option: " + k);
@@ -685,6 +718,48 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     a.p1_f = c->dp1_f;
     if (c->family == OJMI_KERNEL_SQA_WAVE) {
       e = launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
+    } else if (c->family == OJMI_KERNEL_DENSE_SHARED) {
+      SharedArgs sa{};
+      sa.J = c->dJnat ? c->dJnat : c->dJ;
+      sa.h = c->dh;
+      sa.spins = c->dspins;
+      sa.n = (int)c->n;
+      sa.R = (int)c->R;
+      sa.T = (int)T;
+      sa.seed = seed;
+      sa.t0 = sweep_offset;
+      sa.rep_offset = 0;
+      sa.p0_d = c->dp0_d;
+      sa.p0_f = c->dp0_f;
+      if (c->ftype == FT_F32) {
+        if (!c->dF) {
+          float* f;
+          if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
+          c->dF = f;
+        }
+        sa.F_out = c->dF;
+      }
+      const int G = c->group;
+      const size_t fixed = shared_lds_bytes((int)c->NS, G, 0, 4);
+      const size_t per_row = (size_t)4 * 64 * c->NS + 4;
+      int rb = (int)std::min<size_t>(64, ((size_t)c->lds_budget - fixed) / per_row);
+      if (rb < 1) return fail(OJMI_ENOTSUP, "shared kernel: LDS too small for one row");
+      sa.ring_rows = rb;
+      const int64_t nwg = (c->R + G - 1) / G;
+      sa.xcd_window = (nwg >= 16) ? c->xcd_window : 0;
+      sa.xcd_spin = 4000;
+      if (sa.xcd_window > 0) {
+        if (c->prog_cap < nwg) {
+          if (c->dprog) (void)hipFree(c->dprog);
+          c->dprog = nullptr;
+          if ((rc = dalloc(&c->dprog, (size_t)nwg))) return rc;
+          c->prog_cap = nwg;
+        }
+        OJMI_HIP(hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st));
+        sa.progress = c->dprog;
+      }
+      e = launch_dense_shared(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
+      c->fields_valid = (c->ftype == FT_F32);
     } else {
       if (c->ftype == FT_F32) {
         if (!c->dF) {

@@ -46,6 +46,29 @@ struct CsrArgs {
   const float* p0_f;
 };
 
+// Workgroup-shared dense sweep (k_shared.hip): J in natural row layout
+// (element j of row i at i*Npad + j), G replicas per workgroup.
+struct SharedArgs {
+  const void* J;
+  const void* h;
+  int8_t* spins;
+  int n, R, T;
+  uint64_t seed, t0;
+  int rep_offset;
+  const double* p0_d;
+  const float* p0_f;
+  const void* F_in;
+  void* F_out;
+  int ring_rows;        // coupling rows staged per LDS round
+  int xcd_window;       // 0: no pacing; else max lead (blocks) over XCD peers
+  int xcd_spin;         // bound on pacing polls per block
+  uint32_t* progress;   // [workgroups] blocks completed
+};
+
+size_t shared_lds_bytes(int NS, int G, int ring_rows, int fbytes);
+hipError_t launch_dense_shared(int ftype, int jtype, const SharedArgs& a, int NS, int G,
+                               hipStream_t st);
+
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, hipStream_t st);

@@ -151,19 +151,17 @@ def test_fixture_ground_states_are_brute_force_minima(fixtures):
 # --- reference restatement ---------------------------------------------------------
 def test_ref_sa_finds_fixture_ground_states(fixtures):
     """SingleSpinFlip.FindTrueGroundState_ClassicalIsing_Dense with the test
-    schedule (beta 0.1 -> 100, 100 levels x 100 sweeps)."""
+    schedule (beta 0.1 -> 100, 100 levels x 100 sweeps): the ground state is
+    the most frequent outcome on every fixture (>= 60% of 40 reads)."""
     c = fixtures["classical_schedule"]
     beta = orc.classical_schedule(c["beta_min"], c["beta_max"], c["num_call_updater"])
     mcs = np.full(len(beta), c["one_mc_step"])
     for k, case in fixtures["testcases"].items():
         E = H.symmetric_ext(_case_ext(case))
-        hits = 0
-        for seed in range(5):
-            init = orc.ref_gen_spin(8, 1234 + seed)
-            out = orc.ref_sa_dense(E, init, 5678 + seed, beta, mcs)
-            hits += list(out) == case["ground_state"]
-        # the "Hard Instance" (case 3, gap 0.2) is found less often
-        assert hits >= (2 if case["hard"] else 4), (k, hits)
+        init = np.array([orc.ref_gen_spin(8, 1234 + s) for s in range(40)])
+        out = orc.ref_sa_dense_batch(E, init, np.arange(40, dtype=np.uint32) + 5678, beta, mcs)
+        hits = sum(list(s) == case["ground_state"] for s in out)
+        assert hits >= 24, (k, hits)
 
 
 def test_ref_sa_csr_equals_dense_for_integer_couplings():
