@@ -123,6 +123,8 @@ def main():
     ap.add_argument("--group", type=int, default=None, help="replicas per workgroup")
     ap.add_argument("--xcd-window", type=int, default=None)
     ap.add_argument("--refresh-every", type=int, default=None)
+    ap.add_argument("--ring-rows", type=int, default=None)
+    ap.add_argument("--round-rows", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; replicas would shard across GPUs "
@@ -145,6 +147,10 @@ def main():
         opts["xcd_window"] = args.xcd_window
     if args.refresh_every is not None:
         opts["refresh_every"] = args.refresh_every
+    if args.ring_rows is not None:
+        opts["ring_rows"] = args.ring_rows
+    if args.round_rows is not None:
+        opts["round_rows"] = args.round_rows
     walls, kms, e_fast, drift, info = time_path(W, W["fast"], args.steps, args.warmup, 1000,
                                                 opts)
     step_s = float(walls.mean())
@@ -183,7 +189,9 @@ def main():
         "data": "synthetic (seeded splitmix64 instance)",
         "config": {"workload": W["name"], "replicas": W["R"], "spins": W["n"],
                    "sweeps": int(W["mcs"].sum()), "path": W["fast"],
-                   "kernel": info["kernel"], "refresh_every": info["refresh_every"]},
+                   "kernel": info["kernel"], "refresh_every": info["refresh_every"],
+                   "group": info["group"], "ring_rows": info["ring_rows"],
+                   "lds_bytes": info["lds_bytes"], "lds_budget": info["lds_budget"]},
         "tts99_s": tts,
         "tts99_success_prob": ps,
         "plain_fp64": None if plain is None else {

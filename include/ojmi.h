@@ -69,6 +69,10 @@ typedef struct ojmi_info {
   int64_t last_attempts;  /* flip attempts of the last ojmi_run              */
   double last_kernel_ms;  /* device time of the last ojmi_run (HIP events)   */
   double last_refresh_ms; /* part of it spent in field refreshes             */
+  int32_t group;          /* replicas per workgroup (1 = one wave each)      */
+  int32_t ring_rows;      /* coupling rows per LDS staging round (shared)    */
+  int32_t lds_bytes;      /* LDS per workgroup of the last launch            */
+  int32_t lds_budget;     /* device LDS limit per workgroup                  */
 } ojmi_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -105,7 +109,11 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
  * Tuning knobs (no reference counterpart).  Keys:
  *   "refresh_every": FP32 path, sweeps between FP32 field refreshes (0 = fields
  *                    are accumulated from the start of each run only)
- *   "group":         replicas per workgroup sharing coupling rows (FP32 path)
+ *   "group":         replicas per workgroup sharing coupling rows (1, 4, 8, 16;
+ *                    classical FP32 / INT32 dense systems)
+ *   "xcd_window":    shared kernel: pace workgroups of one XCD to within this
+ *                    many 64-spin blocks of each other (0 = off)
+ *   "ring_rows":     shared kernel: coupling rows per LDS staging round (0 = auto)
  */
 int ojmi_set_option(ojmi_ctx* ctx, const char* key, int64_t value);
 

@@ -26,7 +26,8 @@ class Info(C.Structure):
         ("precision", C.c_int32), ("kernel", C.c_int32), ("refresh_every", C.c_int32),
         ("coupling_bytes", C.c_int32), ("last_sweeps", C.c_int64),
         ("last_attempts", C.c_int64), ("last_kernel_ms", C.c_double),
-        ("last_refresh_ms", C.c_double),
+        ("last_refresh_ms", C.c_double), ("group", C.c_int32), ("ring_rows", C.c_int32),
+        ("lds_bytes", C.c_int32), ("lds_budget", C.c_int32),
     ]
 
 

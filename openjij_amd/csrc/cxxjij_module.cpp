@@ -690,6 +690,10 @@ struct Engine {
     d["last_attempts"] = i.last_attempts;
     d["last_kernel_ms"] = i.last_kernel_ms;
     d["last_refresh_ms"] = i.last_refresh_ms;
+    d["group"] = i.group;
+    d["ring_rows"] = i.ring_rows;
+    d["lds_bytes"] = i.lds_bytes;
+    d["lds_budget"] = i.lds_budget;
     return d;
   }
 };

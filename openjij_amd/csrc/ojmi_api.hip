@@ -68,6 +68,7 @@ struct ojmi_ctx {
   int group = 1;         // 1: one wave per replica; 4/8/16: replicas per workgroup
   int xcd_window = 2;    // shared kernel: XCD pacing window in blocks (0 = off)
   int lds_budget = 65536;
+  int ring_rows_opt = 0, round_rows_opt = 0, ring_rows_used = 0, lds_used = 0;
   void* dJnat = nullptr;   // natural-layout couplings when dJ is packed
   uint32_t* dprog = nullptr;
   int64_t prog_cap = 0;
@@ -155,11 +156,13 @@ int resolve_precision(int32_t req, bool integer, double max_abs_field, int* out)
 }
 
 int common_init(ojmi_ctx* c) {
-  int dev = 0, lds = 65536;
+  int dev = 0, lds = 0, optin = 0;
   OJMI_HIP(hipGetDevice(&dev));
-  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
-      lds > 0)
-    c->lds_budget = lds;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    lds = 0;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess)
+    optin = 0;
+  c->lds_budget = std::max(65536, std::max(lds, optin));
   OJMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   OJMI_HIP(hipEventCreate(&c->e0));
   OJMI_HIP(hipEventCreate(&c->e1));
@@ -591,8 +594,20 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
       return fail(OJMI_EINVAL, "group > 1 needs a classical FP32 or INT32 system");
     if (c->family != OJMI_KERNEL_DENSE_WAVE && c->family != OJMI_KERNEL_DENSE_SHARED)
       return fail(OJMI_EINVAL, "group applies to dense classical systems");
+    if (value == 16 && c->NS > 32)
+      return fail(OJMI_EINVAL, "group 16 supports n <= 2048 (register budget)");
     c->group = (int)value;
     c->family = value > 1 ? OJMI_KERNEL_DENSE_SHARED : OJMI_KERNEL_DENSE_WAVE;
+    return OJMI_OK;
+  }
+  if (k == "ring_rows") {
+    if (value < 0 || value > 64) return fail(OJMI_EINVAL, "ring_rows must be in 0..64");
+    c->ring_rows_opt = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "round_rows") {
+    if (value < 0 || value > 32) return fail(OJMI_EINVAL, "round_rows must be in 0..32");
+    c->round_rows_opt = (int)value;
     return OJMI_OK;
   }
   if (k == "xcd_window") {
@@ -740,11 +755,19 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
         sa.F_out = c->dF;
       }
       const int G = c->group;
-      const size_t fixed = shared_lds_bytes((int)c->NS, G, 0, 4);
-      const size_t per_row = (size_t)4 * 64 * c->NS + 4;
+      const int jb = (c->jtype == JT_I8) ? 1 : 4;
+      const size_t fixed = shared_lds_bytes((int)c->NS, G, 0, jb);
+      const size_t per_row = (size_t)jb * 64 * c->NS;
       int rb = (int)std::min<size_t>(64, ((size_t)c->lds_budget - fixed) / per_row);
-      if (rb < 1) return fail(OJMI_ENOTSUP, "shared kernel: LDS too small for one row");
+      if (c->ring_rows_opt > 0) rb = std::min(rb, c->ring_rows_opt);
+      // rounds of S rows, at least two buffers (one consumed, others in flight)
+      const int S = std::max(1, std::min(c->round_rows_opt > 0 ? c->round_rows_opt : 2, rb / 2));
+      rb = (rb / S) * S;
+      if (rb < 2 * S) return fail(OJMI_ENOTSUP, "shared kernel: LDS too small for two rows");
       sa.ring_rows = rb;
+      sa.round_rows = S;
+      c->ring_rows_used = rb;
+      c->lds_used = (int)shared_lds_bytes((int)c->NS, G, rb, jb);
       const int64_t nwg = (c->R + G - 1) / G;
       sa.xcd_window = (nwg >= 16) ? c->xcd_window : 0;
       sa.xcd_spin = 4000;
@@ -852,6 +875,10 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->last_attempts = c->last_attempts;
   info->last_kernel_ms = c->last_ms;
   info->last_refresh_ms = c->last_refresh_ms;
+  info->group = c->group;
+  info->ring_rows = c->ring_rows_used;
+  info->lds_bytes = c->lds_used;
+  info->lds_budget = c->lds_budget;
   return OJMI_OK;
 }
 

@@ -59,7 +59,8 @@ struct SharedArgs {
   const float* p0_f;
   const void* F_in;
   void* F_out;
-  int ring_rows;        // coupling rows staged per LDS round
+  int ring_rows;        // coupling rows held in the LDS ring (multiple of round_rows)
+  int round_rows;       // rows per staging round (ring_rows / round_rows >= 2 buffers)
   int xcd_window;       // 0: no pacing; else max lead (blocks) over XCD peers
   int xcd_spin;         // bound on pacing polls per block
   uint32_t* progress;   // [workgroups] blocks completed

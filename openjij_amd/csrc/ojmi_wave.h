@@ -29,12 +29,15 @@ struct FieldVec {
 
   __device__ __forceinline__ FT get(int m) const { return v[m / VL][m % VL]; }
   __device__ __forceinline__ void set(int m, FT x) { v[m / VL][m % VL] = x; }
-  // k is wave-uniform
+  // k is wave-uniform.  readfirstlane makes that explicit to the compiler so
+  // the access becomes s_set_gpr_idx indirect addressing on a scalar index
+  // (without it, index arithmetic placed in VGPRs turns into a waterfall loop).
   __device__ __forceinline__ FT dyn(int k) const {
+    const int ku = __builtin_amdgcn_readfirstlane(k);
     if constexpr (NV == 1) {
-      return v[0][k];
+      return v[0][ku];
     } else {
-      const int q = k / VL, e = k - q * VL;
+      const int q = ku / VL, e = ku - q * VL;
       FT r = v[0][e];
 #pragma unroll
       for (int qq = 1; qq < NV; ++qq)
