@@ -125,6 +125,7 @@ def main():
     ap.add_argument("--refresh-every", type=int, default=None)
     ap.add_argument("--ring-rows", type=int, default=None)
     ap.add_argument("--round-rows", type=int, default=None)
+    ap.add_argument("--colown", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; replicas would shard across GPUs "
@@ -151,6 +152,8 @@ def main():
         opts["ring_rows"] = args.ring_rows
     if args.round_rows is not None:
         opts["round_rows"] = args.round_rows
+    if args.colown is not None:
+        opts["colown"] = args.colown
     walls, kms, e_fast, drift, info = time_path(W, W["fast"], args.steps, args.warmup, 1000,
                                                 opts)
     step_s = float(walls.mean())

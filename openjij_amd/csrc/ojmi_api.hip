@@ -69,6 +69,7 @@ struct ojmi_ctx {
   int xcd_window = 2;    // shared kernel: XCD pacing window in blocks (0 = off)
   int lds_budget = 65536;
   int ring_rows_opt = 0, round_rows_opt = 0, ring_rows_used = 0, lds_used = 0;
+  int colown = 1;  // shared family: 1 = column-owned kernel when the shape allows
   void* dJnat = nullptr;   // natural-layout couplings when dJ is packed
   uint32_t* dprog = nullptr;
   int64_t prog_cap = 0;
@@ -605,6 +606,11 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     c->ring_rows_opt = (int)value;
     return OJMI_OK;
   }
+  if (k == "colown") {
+    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "colown must be 0 or 1");
+    c->colown = (int)value;
+    return OJMI_OK;
+  }
   if (k == "round_rows") {
     if (value < 0 || value > 32) return fail(OJMI_EINVAL, "round_rows must be in 0..32");
     c->round_rows_opt = (int)value;
@@ -756,6 +762,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
       }
       const int G = c->group;
       const int jb = (c->jtype == JT_I8) ? 1 : 4;
+      const bool use_colown = c->colown && colown_supported((int)c->NS, G);
       const size_t fixed = shared_lds_bytes((int)c->NS, G, 0, jb);
       const size_t per_row = (size_t)jb * 64 * c->NS;
       int rb = (int)std::min<size_t>(64, ((size_t)c->lds_budget - fixed) / per_row);
@@ -781,7 +788,13 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
         OJMI_HIP(hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st));
         sa.progress = c->dprog;
       }
-      e = launch_dense_shared(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
+      if (use_colown) {
+        c->ring_rows_used = 0;
+        c->lds_used = (int)colown_lds_bytes((int)c->n, (int)c->NS, G, jb, 4);
+        e = launch_dense_colown(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
+      } else {
+        e = launch_dense_shared(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
+      }
       c->fields_valid = (c->ftype == FT_F32);
     } else {
       if (c->ftype == FT_F32) {

@@ -70,6 +70,11 @@ size_t shared_lds_bytes(int NS, int G, int ring_rows, int fbytes);
 hipError_t launch_dense_shared(int ftype, int jtype, const SharedArgs& a, int NS, int G,
                                hipStream_t st);
 
+size_t colown_lds_bytes(int n, int NS, int G, int jbytes, int fbytes);
+bool colown_supported(int NS, int G);
+hipError_t launch_dense_colown(int ftype, int jtype, const SharedArgs& a, int NS, int G,
+                               hipStream_t st);
+
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, hipStream_t st);

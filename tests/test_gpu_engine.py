@@ -262,18 +262,22 @@ def test_energy_distribution_matches_reference_ks():
 
 
 # --- workgroup-shared dense kernel ---------------------------------------------
+@pytest.mark.parametrize("colown", [0, 1])
 @pytest.mark.parametrize("n,R,G,prec", [(64, 8, 8, "fp32"), (100, 13, 4, "fp32"),
                                         (257, 40, 8, "fp32"), (1000, 33, 16, "fp32"),
                                         (300, 24, 8, "int32"), (129, 10, 4, "int32"),
-                                        (4096, 16, 8, "fp32")])
-def test_shared_kernel_bitexact(n, R, G, prec):
-    """Replicas of a workgroup share accepted coupling rows; the trajectory is
-    still the sequential one (oracle replay, bit for bit)."""
+                                        (4096, 16, 8, "fp32"), (600, 11, 8, "fp32"),
+                                        (520, 9, 4, "int32"), (2048, 20, 4, "fp32")])
+def test_shared_kernel_bitexact(n, R, G, prec, colown):
+    """Replicas of a workgroup share accepted coupling rows (rows staged in LDS,
+    or column-owned fields with each row read once per workgroup); the
+    trajectory is still the sequential one (oracle replay, bit for bit)."""
     E = H.sk_gauss(n, 9000 + n, field=True) if prec == "fp32" else H.int_dense(n, 9000 + n)
     beta, mcs = sched(12 if n >= 4096 else 25, 0.1, 6.0)
     init = H.random_pm1(R, n, 5)
     with capi.Context.dense(E, R, 1, 0.0, PREC[prec]) as ctx:
         ctx.set_option("group", G)
+        ctx.set_option("colown", colown)
         assert ctx.info()["kernel"] == capi.KERNEL_DENSE_SHARED
         ctx.set_spins(init)
         ctx.run(beta, mcs, 777, 4)
@@ -283,14 +287,16 @@ def test_shared_kernel_bitexact(n, R, G, prec):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("colown", [0, 1])
 @pytest.mark.parametrize("window", [0, 1, 3])
-def test_shared_kernel_xcd_pacing_does_not_change_results(window):
+def test_shared_kernel_xcd_pacing_does_not_change_results(window, colown):
     n, R = 512, 256  # 32 workgroups of 8: pacing active when window > 0
     E = H.sk_gauss(n, 31)
     beta, mcs = sched(10, 0.1, 6.0)
     init = H.random_pm1(R, n, 6)
     with capi.Context.dense(E, R, 1, 0.0, capi.PREC_FP32) as ctx:
         ctx.set_option("group", 8)
+        ctx.set_option("colown", colown)
         ctx.set_option("xcd_window", window)
         ctx.set_spins(init)
         ctx.run(beta, mcs, 99, 0)
