@@ -54,6 +54,7 @@ extern "C" {
 #define OJMI_KERNEL_SQA_WAVE 2     /* one wave64 per Trotter slice              */
 #define OJMI_KERNEL_CSR_LEVEL 3    /* one wave64 per replica, level-parallel   */
 #define OJMI_KERNEL_DENSE_SHARED 4 /* replicas of a workgroup share J rows     */
+#define OJMI_KERNEL_DENSE_COLOWN 5 /* as 4, each wave owns a slice of the fields */
 
 typedef struct ojmi_ctx ojmi_ctx;
 
@@ -114,6 +115,9 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
  *   "xcd_window":    shared kernel: pace workgroups of one XCD to within this
  *                    many 64-spin blocks of each other (0 = off)
  *   "ring_rows":     shared kernel: coupling rows per LDS staging round (0 = auto)
+ *   "round_rows":    shared kernel: rows per staging round (0 = auto)
+ *   "colown":        1 (default) = run shared contexts with the column-owned
+ *                    kernel where the shape allows it, 0 = LDS row staging
  */
 int ojmi_set_option(ojmi_ctx* ctx, const char* key, int64_t value);
 

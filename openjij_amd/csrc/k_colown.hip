@@ -356,7 +356,11 @@ static hipError_t launch_colown_t(const SharedArgs& a, hipStream_t st) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid((a.R + G - 1) / G), block(64 * G);
-  hipLaunchKernelGGL(kern, grid, block, lds, st, a);
+  SharedArgs b = a;
+  if (b.xcd_window > 0 && !all_resident(reinterpret_cast<const void*>(kern), (int)grid.x,
+                                         (int)block.x, lds))
+    b.xcd_window = 0;  // pacing waits on peers that may not have started
+  hipLaunchKernelGGL(kern, grid, block, lds, st, b);
   return hipGetLastError();
 }
 

@@ -410,6 +410,16 @@ __global__ __launch_bounds__(64 * G) void k_dense_shared(SharedArgs a) {
   }
 }
 
+bool all_resident(const void* kern, int blocks, int threads, size_t lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return false;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return false;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, lds) != hipSuccess)
+    return false;
+  return (int64_t)per_cu * cus >= blocks;
+}
+
 // LDS of k_dense_shared: two diagonal tiles, the row ring (ring_rows rows of
 // 64*NS couplings of jbytes each) and the per-wave flip words.
 size_t shared_lds_bytes(int NS, int G, int ring_rows, int jbytes) {
@@ -424,7 +434,11 @@ static hipError_t launch_shared_t(const SharedArgs& a, hipStream_t st) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid((a.R + G - 1) / G), block(64 * G);
-  hipLaunchKernelGGL(kern, grid, block, lds, st, a);
+  SharedArgs b = a;
+  if (b.xcd_window > 0 && !all_resident(reinterpret_cast<const void*>(kern), (int)grid.x,
+                                         (int)block.x, lds))
+    b.xcd_window = 0;  // pacing waits on peers that may not have started
+  hipLaunchKernelGGL(kern, grid, block, lds, st, b);
   return hipGetLastError();
 }
 

@@ -377,7 +377,9 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
     return rc;
   }
   if (c->M == 1 && c->ftype == FT_F32 && c->R >= 8) {
-    c->group = 8;
+    // column-owned fields with 4 replicas per workgroup where the row count
+    // allows it, else rows staged in LDS for 8 replicas
+    c->group = colown_supported((int)c->NS, 4) ? 4 : 8;
     c->family = OJMI_KERNEL_DENSE_SHARED;
   }
   *out = c;
@@ -787,6 +789,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
         }
         OJMI_HIP(hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st));
         sa.progress = c->dprog;
+        OJMI_HIP(hipEventRecord(c->e0, c->st));  // time the sweep kernel only
       }
       if (use_colown) {
         c->ring_rows_used = 0;
@@ -882,6 +885,8 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->num_trotter = c->M;
   info->precision = c->prec;
   info->kernel = c->family;
+  if (c->family == OJMI_KERNEL_DENSE_SHARED && c->colown && colown_supported((int)c->NS, c->group))
+    info->kernel = OJMI_KERNEL_DENSE_COLOWN;
   info->refresh_every = c->refresh_every;
   info->coupling_bytes = (c->jtype == JT_F64) ? 8 : (c->jtype == JT_I8 ? 1 : 4);
   info->last_sweeps = c->last_T;

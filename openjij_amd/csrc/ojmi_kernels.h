@@ -67,6 +67,9 @@ struct SharedArgs {
 };
 
 size_t shared_lds_bytes(int NS, int G, int ring_rows, int fbytes);
+// True when every workgroup of a grid of `blocks` can be resident on the device
+// at once (occupancy of `kern` at this block size and LDS times the CU count).
+bool all_resident(const void* kern, int blocks, int threads, size_t lds);
 hipError_t launch_dense_shared(int ftype, int jtype, const SharedArgs& a, int NS, int G,
                                hipStream_t st);
 

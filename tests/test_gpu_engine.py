@@ -278,7 +278,10 @@ def test_shared_kernel_bitexact(n, R, G, prec, colown):
     with capi.Context.dense(E, R, 1, 0.0, PREC[prec]) as ctx:
         ctx.set_option("group", G)
         ctx.set_option("colown", colown)
-        assert ctx.info()["kernel"] == capi.KERNEL_DENSE_SHARED
+        slots = 1 << max(0, (n + 63) // 64 - 1).bit_length()
+        owned = colown and G in (4, 8) and slots >= 8 and slots % G == 0
+        assert ctx.info()["kernel"] == (capi.KERNEL_DENSE_COLOWN if owned
+                                        else capi.KERNEL_DENSE_SHARED)
         ctx.set_spins(init)
         ctx.run(beta, mcs, 777, 4)
         got = ctx.get_spins()
