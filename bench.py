@@ -257,7 +257,6 @@ def main():
     ap.add_argument("--replicas", type=int, default=None)
     ap.add_argument("--group", type=int, default=None, help="replicas per workgroup")
     ap.add_argument("--xcd-window", type=int, default=None)
-    ap.add_argument("--refresh-every", type=int, default=None)
     ap.add_argument("--ring-rows", type=int, default=None)
     ap.add_argument("--round-rows", type=int, default=None)
     ap.add_argument("--colown", type=int, default=None)
@@ -280,7 +279,7 @@ def main():
     log(f"workload: {W['name']} ({attempts:.3e} flip attempts per step)")
 
     opts = {}
-    for key in ("group", "xcd_window", "refresh_every", "ring_rows", "round_rows", "colown"):
+    for key in ("group", "xcd_window", "ring_rows", "round_rows", "colown"):
         v = getattr(args, key)
         if v is not None:
             opts[key] = v
@@ -310,7 +309,9 @@ def main():
 
     kname = KERNEL_NAMES.get(info["kernel"], str(info["kernel"]))
     bJ = info["coupling_bytes"]
-    b_alg = W["deg"] * bJ  # one coupling row (deg couplings) consulted per attempt
+    # one coupling row (deg couplings, plus an int32 column index each for CSR)
+    # consulted per attempt
+    b_alg = W["deg"] * (bJ + (4 if W["kind"] == "csr" else 0))
     achieved = attempts * b_alg / (kernel_ms * 1e-3)
     default_run = (not opts and args.sweeps is None and args.replicas is None)
     traffic = profiled_traffic(kname, W["name"]) if default_run else None

@@ -64,7 +64,7 @@ typedef struct ojmi_info {
   int64_t num_trotter;    /* M (1 for classical)                             */
   int32_t precision;      /* resolved OJMI_PREC_*                            */
   int32_t kernel;         /* OJMI_KERNEL_*                                   */
-  int32_t refresh_every;  /* FP32 path: sweeps between field refreshes       */
+  int32_t refresh_every;  /* sweeps between field refreshes (always 0: none) */
   int32_t coupling_bytes; /* stored bytes per coupling (1, 4 or 8)           */
   int64_t last_sweeps;    /* expanded sweeps of the last ojmi_run            */
   int64_t last_attempts;  /* flip attempts of the last ojmi_run              */
@@ -108,8 +108,9 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
 
 /*
  * Tuning knobs (no reference counterpart).  Keys:
- *   "refresh_every": FP32 path, sweeps between FP32 field refreshes (0 = fields
- *                    are accumulated from the start of each run only)
+ *   "refresh_every": sweeps between field refreshes; only 0 (fields are built
+ *                    at the start of each run and updated per flip) is
+ *                    accepted, other values return OJMI_ENOTSUP
  *   "group":         replicas per workgroup sharing coupling rows (1, 4, 8, 16;
  *                    classical FP32 / INT32 dense systems)
  *   "xcd_window":    shared kernel: pace workgroups of one XCD to within this

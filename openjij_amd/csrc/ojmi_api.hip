@@ -587,7 +587,10 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   const std::string k(key);
   if (k == "refresh_every") {
     if (value < 0) return fail(OJMI_EINVAL, "refresh_every must be >= 0");
-    c->refresh_every = (int)value;
+    // FP32 fields take one rounding per accepted flip in sweep order (replayed
+    // exactly by the CPU replay); no kernel recomputes them mid-run.
+    if (value > 0) return fail(OJMI_ENOTSUP, "periodic field refresh is not implemented");
+    c->refresh_every = 0;
     return OJMI_OK;
   }
   if (k == "group") {
