@@ -3,8 +3,9 @@
 //
 // Mirrors the Python-visible names of the reference binding
 // (openjij/main.cpp:36-226, openjij/declare.hpp) for:
-//   graph:     Graph, Dense, CSRSparse                 (declare.hpp:43-129,196-217)
-//   system:    ClassicalIsing_{Dense,CSRSparse}, TransverseIsing_{Dense,CSRSparse},
+//   graph:     Graph, Dense, Sparse, CSRSparse         (declare.hpp:43-217)
+//   system:    ClassicalIsing_{Dense,Sparse,CSRSparse},
+//              TransverseIsing_{Dense,Sparse,CSRSparse},
 //              make_classical_ising, make_transverse_ising (declare.hpp:409-438,585-641)
 //   algorithm: Algorithm_SingleSpinFlip_run, four overloads per system (declare.hpp:687-793)
 //   utility:   ClassicalUpdaterParameter, TransverseFieldUpdaterParameter,
@@ -24,6 +25,8 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
+#include <map>
 #include <memory>
 #include <random>
 #include <stdexcept>
@@ -204,6 +207,76 @@ struct CSRGraph : Graph {
   }
 };
 
+// Adjacency-list graph (include/openjij/graph/sparse.hpp:60-316): J(i,j) and
+// h(i) = J(i,i) register an edge in both adjacency lists, at most num_edges
+// per node; a coupling whose edge did not fit is stored but, as in the
+// reference's release build, not part of the system's matrix.
+struct SparseGraph : Graph {
+  size_t num_edges;
+  std::vector<std::vector<size_t>> adj;
+  std::map<std::pair<size_t, size_t>, double> Jm;
+  SparseGraph(size_t num, size_t edges) : Graph(num), num_edges(std::min(num, edges)), adj(num) {}
+  explicit SparseGraph(size_t num) : SparseGraph(num, num) {}
+  void check_index(size_t i) const {
+    if (i >= n) throw std::out_of_range("index out of range");
+  }
+  bool set_adj(size_t from, size_t to) {
+    auto& nodes = adj[from];
+    if (std::find(nodes.begin(), nodes.end(), to) == nodes.end()) {
+      if (nodes.size() >= num_edges) return false;
+      nodes.push_back(to);
+      set_adj(to, from);
+    }
+    return true;
+  }
+  double& J(size_t i, size_t j) {
+    check_index(i);
+    check_index(j);
+    set_adj(i, j);
+    return Jm[std::make_pair(std::min(i, j), std::max(i, j))];
+  }
+  double Jget(size_t i, size_t j) const {
+    check_index(i);
+    check_index(j);
+    auto it = Jm.find(std::make_pair(std::min(i, j), std::max(i, j)));
+    if (it == Jm.end()) throw std::out_of_range("map::at");
+    return it->second;
+  }
+  // include/openjij/graph/sparse.hpp:231-250
+  double energy(const std::vector<double>& s) const {
+    if (s.size() != n) throw std::out_of_range("Out of range in energy in Sparse graph.");
+    double e = 0;
+    for (size_t i = 0; i < n; ++i)
+      for (size_t j : adj[i]) {
+        if (i != j) e += 0.5 * Jget(i, j) * s[i] * s[j];
+        else e += Jget(i, i) * s[i];
+      }
+    return e;
+  }
+  // the system matrix of include/openjij/utility/eigen.hpp:162-191, upper triangle
+  CSRGraph csr() const {
+    CSRGraph g(n);
+    g.rowptr.assign(1, 0);
+    for (size_t i = 0; i < n; ++i) {
+      std::vector<std::pair<int64_t, double>> row;
+      for (size_t j : adj[i]) {
+        if (j > i) row.push_back({(int64_t)j, Jget(i, j)});
+        else if (j == i) row.push_back({(int64_t)n, Jget(i, i)});
+      }
+      std::sort(row.begin(), row.end());
+      for (auto& p : row) {
+        g.col.push_back(p.first);
+        g.val.push_back(p.second);
+      }
+      g.rowptr.push_back((int64_t)g.col.size());
+    }
+    g.col.push_back((int64_t)n);
+    g.val.push_back(1.0);
+    g.rowptr.push_back((int64_t)g.col.size());
+    return g;
+  }
+};
+
 std::shared_ptr<CSRGraph> csr_from_scipy(py::object mat) {
   py::object csr = mat.attr("tocsr")();
   py::tuple shape = csr.attr("shape");
@@ -338,6 +411,10 @@ ojmi_ctx* make_ctx(const CSRGraph& g, int64_t M, double gamma) {
   check(ojmi_create_csr((int64_t)g.n, g.rowptr.data(), g.col.data(), g.val.data(), 1, M, gamma,
                         OJMI_PREC_AUTO, &c));
   return c;
+}
+
+ojmi_ctx* make_ctx(const SparseGraph& g, int64_t M, double gamma) {
+  return make_ctx(g.csr(), M, gamma);
 }
 
 template <typename G>
@@ -539,6 +616,8 @@ void declare_systems(py::module_& msys, py::module_& malg, py::module_& mres,
                                  py::array_t<double> a({s.num_spins + 1, s.num_spins + 1});
                                  std::copy(v.begin(), v.end(), a.mutable_data());
                                  return std::move(a);
+                               } else if constexpr (std::is_same<G, SparseGraph>::value) {
+                                 return csr_to_scipy_symmetric(s.graph->csr());
                                } else {
                                  return csr_to_scipy_symmetric(*s.graph);
                                }
@@ -594,6 +673,8 @@ void declare_systems(py::module_& msys, py::module_& malg, py::module_& mres,
                                  py::array_t<double> a({n1, n1});
                                  std::copy(v.begin(), v.end(), a.mutable_data());
                                  return std::move(a);
+                               } else if constexpr (std::is_same<G, SparseGraph>::value) {
+                                 return csr_to_scipy_symmetric(s.graph->csr());
                                } else {
                                  return csr_to_scipy_symmetric(*s.graph);
                                }
@@ -770,6 +851,35 @@ PYBIND11_MODULE(cxxjij, m) {
           "spins"_a)
       .def("get_interactions", [](const CSRGraph& g) { return csr_to_scipy_symmetric(g); });
 
+  py::class_<SparseGraph, Graph, std::shared_ptr<SparseGraph>>(mg, "Sparse", py::module_local())
+      .def(py::init<size_t, size_t>(), "num_spins"_a, "num_edges"_a)
+      .def(py::init<size_t>(), "num_spins"_a)
+      .def(py::init([](const SparseGraph& o) { return std::make_shared<SparseGraph>(o); }),
+           "other"_a)
+      .def("adj_nodes",
+           [](const SparseGraph& g, size_t i) {
+             g.check_index(i);
+             return g.adj[i];
+           })
+      .def("get_num_edges", [](const SparseGraph& g) { return g.num_edges; })
+      .def(
+          "calc_energy",
+          [](const SparseGraph& g, const std::vector<double>& s) { return g.energy(s); },
+          "spins"_a)
+      .def(
+          "__setitem__",
+          [](SparseGraph& g, std::pair<size_t, size_t> k, double v) { g.J(k.first, k.second) = v; },
+          "key"_a, "val"_a)
+      .def(
+          "__getitem__",
+          [](const SparseGraph& g, std::pair<size_t, size_t> k) { return g.Jget(k.first, k.second); },
+          "key"_a)
+      .def(
+          "__setitem__", [](SparseGraph& g, size_t k, double v) { g.J(k, k) = v; }, "key"_a,
+          "val"_a)
+      .def(
+          "__getitem__", [](const SparseGraph& g, size_t k) { return g.Jget(k, k); }, "key"_a);
+
   py::module_ mu = m.def_submodule("utility", "cxxjij module for utility");
   py::class_<ClassicalParam>(mu, "ClassicalUpdaterParameter", py::module_local())
       .def(py::init<>())
@@ -840,6 +950,7 @@ PYBIND11_MODULE(cxxjij, m) {
   py::module_ ma = m.def_submodule("algorithm", "cxxjij module for algorithm");
   py::module_ mr = m.def_submodule("result", "cxxjij module for result");
   declare_systems<DenseGraph>(ms, ma, mr, "_Dense");
+  declare_systems<SparseGraph>(ms, ma, mr, "_Sparse");
   declare_systems<CSRGraph>(ms, ma, mr, "_CSRSparse");
   py::module_ me = m.def_submodule("engine", "batched GPU engine (num_reads replicas per launch)");
   me.attr("PREC_AUTO") = OJMI_PREC_AUTO;

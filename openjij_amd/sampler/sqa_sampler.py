@@ -125,8 +125,9 @@ class SQASampler(BaseSampler):
             tspins = np.zeros((R, M, 0), dtype=np.int8)
             exec_us = np.zeros(R)
         else:
-            eng = cxxjij.engine.Engine(graph, R, M, float(p["gamma"]), _precision_code(precision))
             if reinitialize_state:
+                eng = cxxjij.engine.Engine(graph, R, M, float(p["gamma"]),
+                                           _precision_code(precision))
                 eng.set_spins(init)
                 eng.run(b_arr, s_arr, mcs, seed_v, 0)
                 exec_us = np.full(R, eng.info()["last_kernel_ms"] * 1e3 / R)
