@@ -55,6 +55,7 @@ extern "C" {
 #define OJMI_KERNEL_CSR_LEVEL 3    /* one wave64 per replica, level-parallel   */
 #define OJMI_KERNEL_DENSE_SHARED 4 /* replicas of a workgroup share J rows     */
 #define OJMI_KERNEL_DENSE_COLOWN 5 /* as 4, each wave owns a slice of the fields */
+#define OJMI_KERNEL_BATCH_WAVE 6   /* batch of problems, one wave per replica   */
 
 typedef struct ojmi_ctx ojmi_ctx;
 
@@ -74,6 +75,7 @@ typedef struct ojmi_info {
   int32_t ring_rows;      /* coupling rows per LDS staging round (shared)    */
   int32_t lds_bytes;      /* LDS per workgroup of the last launch            */
   int32_t lds_budget;     /* device LDS limit per workgroup                  */
+  int64_t num_problems;   /* problems of a batch context (1 otherwise)       */
 } ojmi_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -105,6 +107,30 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
 int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const double* val,
                     int64_t num_replicas, int64_t num_trotter, double gamma, int32_t precision,
                     ojmi_ctx** out);
+
+/*
+ * Batch context: num_problems independent classical dense problems of
+ * different sizes n[k] (1..4096), each annealed as num_replicas replicas in
+ * one launch per slot class.  No single reference function corresponds: it
+ * replaces running SASampler.sample / Algorithm<SingleSpinFlip>::run once per
+ * problem and read (openjij/sampler/sampler.py:115-140).  J_ext holds the
+ * problems' (n[k]+1)^2 matrices back to back, each in the ojmi_create_dense
+ * layout (upper triangle read).  Spins and solutions are [k][replica][n[k]]
+ * back to back; ojmi_energies returns [k][replica].  Problem k's random
+ * stream uses k as the fourth counter word (the Trotter slice word of the
+ * single-problem contexts), replica r as the third.
+ */
+int ojmi_create_batch(int64_t num_problems, const int64_t* n, const double* J_ext,
+                      int64_t num_replicas, int32_t precision, ojmi_ctx** out);
+
+/*
+ * Anneal every replica of every problem of a batch context, problem k
+ * through beta[k*L + l] for l = 0..L-1 with one_mc_step[l] sweeps each (the
+ * sweep counts are shared, the temperatures are per problem).  ojmi_run on a
+ * batch context applies the same beta[L] to every problem.
+ */
+int ojmi_run_batch(ojmi_ctx* ctx, const double* beta, const int64_t* one_mc_step, int64_t L,
+                   uint64_t seed, uint64_t sweep_offset);
 
 /*
  * Tuning knobs (no reference counterpart).  Keys:

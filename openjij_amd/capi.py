@@ -18,7 +18,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "ojmi.h")
 OK, EINVAL, ERANGE, EDEVICE, ENOMEM, ENOTSUP, ERUNTIME = range(7)
 PREC_AUTO, PREC_FP64, PREC_FP32, PREC_INT32 = range(4)
 KERNEL_DENSE_WAVE, KERNEL_SQA_WAVE, KERNEL_CSR_LEVEL, KERNEL_DENSE_SHARED = 1, 2, 3, 4
-KERNEL_DENSE_COLOWN = 5
+KERNEL_DENSE_COLOWN, KERNEL_BATCH_WAVE = 5, 6
 
 
 class Info(C.Structure):
@@ -28,7 +28,7 @@ class Info(C.Structure):
         ("coupling_bytes", C.c_int32), ("last_sweeps", C.c_int64),
         ("last_attempts", C.c_int64), ("last_kernel_ms", C.c_double),
         ("last_refresh_ms", C.c_double), ("group", C.c_int32), ("ring_rows", C.c_int32),
-        ("lds_bytes", C.c_int32), ("lds_budget", C.c_int32),
+        ("lds_bytes", C.c_int32), ("lds_budget", C.c_int32), ("num_problems", C.c_int64),
     ]
 
 
@@ -63,6 +63,8 @@ def lib():
             "ojmi_field_drift": (C.c_int, [vp, C.POINTER(dbl)]),
             "ojmi_get_info": (C.c_int, [vp, C.POINTER(Info)]),
             "ojmi_destroy": (C.c_int, [vp]),
+            "ojmi_create_batch": (C.c_int, [i64, vp, vp, i64, i32, C.POINTER(vp)]),
+            "ojmi_run_batch": (C.c_int, [vp, vp, vp, i64, u64, u64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -178,3 +180,54 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+
+class BatchContext(Context):
+    """Batch of K independent dense problems of sizes n[k], R replicas each
+    (ojmi_create_batch).  Spins are a list of [R][n_k] arrays."""
+
+    def __init__(self, handle, ns, R):
+        super().__init__(handle, int(sum(ns)), R, 1)
+        self.ns = [int(x) for x in ns]
+
+    @classmethod
+    def create(cls, J_exts, R, precision=PREC_AUTO):
+        ns = np.array([np.asarray(J).shape[0] - 1 for J in J_exts], dtype=np.int64)
+        flat = np.concatenate([np.ascontiguousarray(J, dtype=np.float64).ravel()
+                               for J in J_exts])
+        h = C.c_void_p()
+        _check(lib().ojmi_create_batch(len(ns), _p(ns), _p(flat), R, precision, C.byref(h)))
+        return cls(h, ns, R)
+
+    def set_spins(self, spins):
+        """spins: list of [R][n_k] +-1 arrays, one per problem."""
+        flat = np.concatenate([np.ascontiguousarray(s, dtype=np.int8).ravel() for s in spins])
+        super().set_spins(flat)
+
+    def _split(self, flat):
+        out, o = [], 0
+        for n in self.ns:
+            out.append(flat[o:o + self.R * n].reshape(self.R, n))
+            o += self.R * n
+        return out
+
+    def get_spins(self):
+        flat = np.empty(self.R * self.n, np.int8)
+        _check(lib().ojmi_get_spins(self.h, _p(flat)))
+        return self._split(flat)
+
+    def solution(self):
+        return self.get_spins()
+
+    def run_batch(self, betas, mcs, seed, sweep_offset=0):
+        """betas: [K][L] per-problem temperatures; mcs: [L] sweeps per entry."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        m = np.ascontiguousarray(mcs, dtype=np.int64)
+        if b.ndim != 2 or b.shape != (len(self.ns), len(m)):
+            raise ValueError("betas must be [num_problems][len(mcs)]")
+        _check(lib().ojmi_run_batch(self.h, _p(b), _p(m), len(m), int(seed), int(sweep_offset)))
+
+    def energies(self):
+        e = np.empty((len(self.ns), self.R), np.float64)
+        _check(lib().ojmi_energies(self.h, _p(e)))
+        return e

@@ -62,18 +62,17 @@ __device__ __forceinline__ void slot_words(uint32_t seed_lo, uint32_t seed_hi, u
 }
 
 // ---------------------------------------------------------------------------
-// Classical: one wave per replica.
+// Classical: one wave anneals one replica through T sweeps.  `rep` and `slice`
+// are the replica and fourth counter words of its random stream; p0[t] is the
+// sweep's threshold factor -1/beta.
 template <typename FT, typename JT, int NS>
-__global__ __launch_bounds__(256) void k_dense_wave(DenseArgs a) {
+__device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_t* sp, int n,
+                                               const FT* F_in, FT* F_out, uint32_t rep,
+                                               uint32_t slice, uint64_t seed, uint64_t t0,
+                                               int T, const typename ThreshT<FT>::type* p0,
+                                               int lane) {
   typedef typename ThreshT<FT>::type TT;
-  const int lane = threadIdx.x & 63;
-  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  if (r >= a.R) return;
-  const int n = a.n;
-  const char* J = reinterpret_cast<const char*>(a.J);
   constexpr int64_t RB = row_bytes<JT, NS>();
-  int8_t* sp = a.spins + (int64_t)r * n;
-
   uint64_t down = 0;
 #pragma unroll
   for (int m = 0; m < NS; ++m) {
@@ -81,23 +80,21 @@ __global__ __launch_bounds__(256) void k_dense_wave(DenseArgs a) {
     if (j < n && sp[j] < 0) down |= 1ull << m;
   }
   FieldVec<FT, NS> F;
-  if (a.F_in) {
-    const FT* fi = reinterpret_cast<const FT*>(a.F_in) + (int64_t)r * (64 * NS);
+  if (F_in) {
 #pragma unroll
-    for (int m = 0; m < NS; ++m) F.set(m, fi[64 * m + lane]);
+    for (int m = 0; m < NS; ++m) F.set(m, F_in[64 * m + lane]);
   } else {
-    init_fields_seq<JT, FT, NS>(F, J, reinterpret_cast<const FT*>(a.h), down, n, lane);
+    init_fields_seq<JT, FT, NS>(F, J, h, down, n, lane);
   }
 
-  const uint32_t rep = (uint32_t)(a.rep_offset + r);
-  const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
+  const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
   constexpr int NQ = (NS + 3) / 4;
-  for (int t = 0; t < a.T; ++t) {
-    const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
-    const TT nb = sweep_param<TT>(a, t);
+  for (int t = 0; t < T; ++t) {
+    const uint32_t tg = (uint32_t)(t0 + (uint64_t)t);
+    const TT nb = p0[t];
     for (int Q = 0; Q < NQ; ++Q) {
       uint32_t w[4];
-      slot_words(slo, shi, tg, rep, 0u, Q, lane, w);
+      slot_words(slo, shi, tg, rep, slice, Q, lane, w);
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
         const int k = 4 * Q + jj;
@@ -128,11 +125,57 @@ __global__ __launch_bounds__(256) void k_dense_wave(DenseArgs a) {
     const int j = 64 * m + lane;
     if (j < n) sp[j] = ((down >> m) & 1ull) ? (int8_t)-1 : (int8_t)1;
   }
-  if (a.F_out) {
-    FT* fo = reinterpret_cast<FT*>(a.F_out) + (int64_t)r * (64 * NS);
+  if (F_out) {
 #pragma unroll
-    for (int m = 0; m < NS; ++m) fo[64 * m + lane] = F.get(m);
+    for (int m = 0; m < NS; ++m) F_out[64 * m + lane] = F.get(m);
   }
+}
+
+template <typename TT>
+__device__ __forceinline__ const TT* p0_ptr(const double* d, const float* f);
+template <>
+__device__ __forceinline__ const double* p0_ptr<double>(const double* d, const float*) {
+  return d;
+}
+template <>
+__device__ __forceinline__ const float* p0_ptr<float>(const double*, const float* f) {
+  return f;
+}
+
+// One wave per replica of one problem.
+template <typename FT, typename JT, int NS>
+__global__ __launch_bounds__(256) void k_dense_wave(DenseArgs a) {
+  typedef typename ThreshT<FT>::type TT;
+  const int lane = threadIdx.x & 63;
+  const int r = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (r >= a.R) return;
+  const FT* F_in = a.F_in ? reinterpret_cast<const FT*>(a.F_in) + (int64_t)r * (64 * NS) : nullptr;
+  FT* F_out = a.F_out ? reinterpret_cast<FT*>(a.F_out) + (int64_t)r * (64 * NS) : nullptr;
+  anneal_replica<FT, JT, NS>(reinterpret_cast<const char*>(a.J), reinterpret_cast<const FT*>(a.h),
+                             a.spins + (int64_t)r * a.n, a.n, F_in, F_out,
+                             (uint32_t)(a.rep_offset + r), 0u, a.seed, a.t0, a.T,
+                             p0_ptr<TT>(a.p0_d, a.p0_f), lane);
+}
+
+// Batch of independent problems of different sizes (the same slot class NS):
+// wave g anneals replica g % R of problem inst[g / R], whose couplings,
+// fields, spins and per-sweep schedule start at its own offsets.  The problem
+// index is the fourth word of the random counter.
+template <typename FT, typename JT, int NS>
+__global__ __launch_bounds__(256) void k_batch_wave(BatchArgs b) {
+  typedef typename ThreshT<FT>::type TT;
+  const int lane = threadIdx.x & 63;
+  const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  if (g >= b.num_items * b.R) return;
+  const int k = b.inst[g / b.R];
+  const int r = g - (g / b.R) * b.R;
+  const int n = b.n[k];
+  const char* J = reinterpret_cast<const char*>(b.J) + b.j_off[k];
+  const FT* h = reinterpret_cast<const FT*>(b.h) + b.h_off[k];
+  int8_t* sp = b.spins + b.s_off[k] + (int64_t)r * n;
+  const TT* p0 = p0_ptr<TT>(b.p0_d, b.p0_f) + (int64_t)k * b.T;
+  anneal_replica<FT, JT, NS>(J, h, sp, n, nullptr, nullptr, (uint32_t)r, (uint32_t)k, b.seed,
+                             b.t0, b.T, p0, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -295,6 +338,39 @@ static hipError_t launch_dense_ft(const DenseArgs& a, int NS, hipStream_t st) {
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <typename FT, typename JT>
+static hipError_t launch_batch_ft(const BatchArgs& b, int NS, hipStream_t st) {
+  const int wpb = 4;
+  const int64_t waves = (int64_t)b.num_items * b.R;
+  dim3 grid((unsigned)((waves + wpb - 1) / wpb)), block(64 * wpb);
+  switch (NS) {
+#define OJMI_CASE(ns)                                                          \
+  case ns:                                                                     \
+    hipLaunchKernelGGL((k_batch_wave<FT, JT, ns>), grid, block, 0, st, b);     \
+    break;
+    OJMI_CASE(1)
+    OJMI_CASE(2)
+    OJMI_CASE(4)
+    OJMI_CASE(8)
+    OJMI_CASE(16)
+    OJMI_CASE(32)
+    OJMI_CASE(64)
+#undef OJMI_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_wave(int ftype, int jtype, const BatchArgs& b, int NS, hipStream_t st) {
+  if (b.num_items <= 0 || b.R <= 0 || b.T <= 0) return hipSuccess;
+  if (ftype == FT_F64 && jtype == JT_F64) return launch_batch_ft<double, double>(b, NS, st);
+  if (ftype == FT_F32 && jtype == JT_F32) return launch_batch_ft<float, float>(b, NS, st);
+  if (ftype == FT_I32 && jtype == JT_I8) return launch_batch_ft<int32_t, int8_t>(b, NS, st);
+  if (ftype == FT_I32 && jtype == JT_I32) return launch_batch_ft<int32_t, int32_t>(b, NS, st);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st) {

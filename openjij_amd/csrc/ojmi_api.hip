@@ -102,13 +102,27 @@ struct ojmi_ctx {
   int64_t last_T = 0, last_attempts = 0;
   double last_ms = 0, last_refresh_ms = 0;
 
+  // batch of independent problems (kind 2): per-problem sizes and offsets
+  int64_t K = 0;
+  std::vector<int32_t> bn;
+  int32_t* d_bn = nullptr;
+  int64_t *d_bj_off = nullptr, *d_bh_off = nullptr, *d_bs_off = nullptr, *d_brb = nullptr;
+  std::vector<std::pair<int, int32_t*>> bclass;  // (slot class NS, device problem list)
+  std::vector<int> bclass_count;
+  double* db_p0_d = nullptr;
+  float* db_p0_f = nullptr;
+  int64_t bTcap = 0;
+
   ~ojmi_ctx() {
     if (dJ == dJ64) dJ = nullptr;
     if (dh == dh64) dh = nullptr;
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
-                    dH, dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog};
+                    dH, dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
+                    d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
+    for (auto& cl : bclass)
+      if (cl.second) (void)hipFree(cl.second);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
@@ -313,6 +327,106 @@ int ensure_sched(ojmi_ctx* c, int64_t T) {
   if ((rc = dalloc(&c->dp0_f, (size_t)T)) != OJMI_OK) return rc;
   if ((rc = dalloc(&c->dp1_f, (size_t)T)) != OJMI_OK) return rc;
   c->Tcap = T;
+  return OJMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch of problems.  Problem k's couplings are stored in the wave kernel's
+// layout for its own slot class NS_k (n_k rows of row_bytes), its fields'
+// constant term as Npad_k entries, its spins as [R][n_k]; all concatenated.
+template <typename JT>
+static void pack_rows(const double* Je, int64_t n, int NS, std::vector<char>& out, size_t off) {
+  const int64_t n1 = n + 1;
+  const int64_t rb = (sizeof(JT) == 1) ? (int64_t)((NS + 3) / 4) * 256 : (int64_t)NS * 64 * sizeof(JT);
+  for (int64_t i = 0; i < n; ++i) {
+    char* row = out.data() + off + (size_t)(i * rb);
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double v = (j > i) ? Je[i * n1 + j] : Je[j * n1 + i];
+      if (sizeof(JT) == 1) {
+        const int64_t m = j >> 6, l = j & 63;
+        row[((m >> 2) * 64 + l) * 4 + (m & 3)] = (char)(int8_t)v;
+      } else {
+        const JT x = (JT)v;
+        std::memcpy(row + j * sizeof(JT), &x, sizeof(JT));
+      }
+    }
+  }
+}
+
+static int64_t batch_row_bytes(int jtype, int NS) {
+  if (jtype == JT_I8) return (int64_t)((NS + 3) / 4) * 256;
+  return (int64_t)NS * 64 * (jtype == JT_F64 ? 8 : 4);
+}
+
+static int run_batch(ojmi_ctx* c, const std::vector<double>& p0 /* [K][T] */, int64_t T,
+                     uint64_t seed, uint64_t sweep_offset) {
+  c->last_T = T;
+  int64_t sum_n = 0;
+  for (int32_t v : c->bn) sum_n += v;
+  c->last_attempts = T * c->R * sum_n;
+  c->last_ms = 0;
+  c->last_refresh_ms = 0;
+  if (T == 0) return OJMI_OK;
+  int rc;
+  if (T > c->bTcap) {
+    if (c->db_p0_d) (void)hipFree(c->db_p0_d);
+    if (c->db_p0_f) (void)hipFree(c->db_p0_f);
+    c->db_p0_d = nullptr;
+    c->db_p0_f = nullptr;
+    c->bTcap = 0;
+    if ((rc = dalloc(&c->db_p0_d, (size_t)(c->K * T))) || (rc = dalloc(&c->db_p0_f, (size_t)(c->K * T))))
+      return rc;
+    c->bTcap = T;
+  }
+  std::vector<float> pf(p0.size());
+  for (size_t q = 0; q < p0.size(); ++q) pf[q] = (float)p0[q];
+  OJMI_HIP(hipMemcpyAsync(c->db_p0_d, p0.data(), p0.size() * 8, hipMemcpyHostToDevice, c->st));
+  OJMI_HIP(hipMemcpyAsync(c->db_p0_f, pf.data(), pf.size() * 4, hipMemcpyHostToDevice, c->st));
+  BatchArgs b{};
+  b.J = c->dJ;
+  b.h = c->dh;
+  b.spins = c->dspins;
+  b.j_off = c->d_bj_off;
+  b.h_off = c->d_bh_off;
+  b.s_off = c->d_bs_off;
+  b.n = c->d_bn;
+  b.R = (int)c->R;
+  b.T = (int)T;
+  b.seed = seed;
+  b.t0 = sweep_offset;
+  b.p0_d = c->db_p0_d;
+  b.p0_f = c->db_p0_f;
+  OJMI_HIP(hipEventRecord(c->e0, c->st));
+  // largest slot class first: the longest waves start first
+  for (size_t q = c->bclass.size(); q-- > 0;) {
+    b.inst = c->bclass[q].second;
+    b.num_items = c->bclass_count[q];
+    hipError_t e = launch_batch_wave(c->ftype, c->jtype, b, c->bclass[q].first, c->st);
+    if (e != hipSuccess)
+      return fail(OJMI_EDEVICE, std::string("batch kernel launch: ") + hipGetErrorString(e));
+  }
+  OJMI_HIP(hipEventRecord(c->e1, c->st));
+  OJMI_HIP(hipEventSynchronize(c->e1));
+  float ms = 0;
+  OJMI_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+  c->last_ms = ms;
+  return OJMI_OK;
+}
+
+static int batch_energies(ojmi_ctx* c) {
+  BatchArgs b{};
+  b.J = c->dJ;
+  b.h = c->dh;
+  b.spins = c->dspins;
+  b.j_off = c->d_bj_off;
+  b.h_off = c->d_bh_off;
+  b.s_off = c->d_bs_off;
+  b.n = c->d_bn;
+  b.R = (int)c->R;
+  hipError_t e = launch_batch_energy(c->ftype, c->jtype, b, c->d_brb, (int)c->K, c->dE, c->st);
+  if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("batch energy: ") + hipGetErrorString(e));
+  OJMI_HIP(hipStreamSynchronize(c->st));
   return OJMI_OK;
 }
 
@@ -582,6 +696,170 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   return OJMI_OK;
 }
 
+int ojmi_create_batch(int64_t num_problems, const int64_t* n, const double* J_ext,
+                      int64_t num_replicas, int32_t precision, ojmi_ctx** out) {
+  if (!out) return fail(OJMI_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (num_problems < 1 || !n || !J_ext) return fail(OJMI_EINVAL, "need >= 1 problem and arrays");
+  if (num_problems > INT32_MAX / 2) return fail(OJMI_EINVAL, "too many problems");
+  if (num_replicas < 1) return fail(OJMI_EINVAL, "num_replicas must be >= 1");
+  const int64_t K = num_problems;
+  std::vector<int64_t> ext_off((size_t)K + 1, 0);
+  int64_t sum_n = 0;
+  for (int64_t k = 0; k < K; ++k) {
+    if (n[k] < 1) return fail(OJMI_EINVAL, "every problem needs n >= 1");
+    if (n[k] > 4096) return fail(OJMI_ENOTSUP, "batch GPU path supports n <= 4096 per problem");
+    ext_off[k + 1] = ext_off[k] + (n[k] + 1) * (n[k] + 1);
+    sum_n += n[k];
+  }
+  if ((double)sum_n * (double)num_replicas > 4.0e9)
+    return fail(OJMI_ERANGE, "batch spins exceed 4e9 entries");
+  bool integer = true;
+  double maxabsJ = 0, maxfield = 0;
+  int rc;
+  for (int64_t k = 0; k < K; ++k) {
+    const double* Je = J_ext + ext_off[k];
+    const int64_t nk = n[k], n1 = nk + 1;
+    if ((rc = check_dense_input(nk, Je))) return fail(rc, "problem " + std::to_string(k) + ": " + g_err);
+    std::vector<double> rs((size_t)nk, 0.0);
+    for (int64_t i = 0; i < nk; ++i) {
+      const double hv = Je[i * n1 + nk];
+      if (hv != std::floor(hv) || std::fabs(hv) > 1e9) integer = false;
+      rs[i] += std::fabs(hv);
+      for (int64_t j = i + 1; j < nk; ++j) {
+        const double v = Je[i * n1 + j];
+        if (v == 0.0) continue;
+        if (v != std::floor(v) || std::fabs(v) > 1e9) integer = false;
+        maxabsJ = std::max(maxabsJ, std::fabs(v));
+        rs[i] += std::fabs(v);
+        rs[j] += std::fabs(v);
+      }
+    }
+    for (double x : rs) maxfield = std::max(maxfield, x);
+  }
+  int prec;
+  if ((rc = resolve_precision(precision, integer, maxfield, &prec))) return rc;
+
+  ojmi_ctx* c = new (std::nothrow) ojmi_ctx;
+  if (!c) return fail(OJMI_ENOMEM, "host allocation failed");
+  c->kind = 2;
+  c->K = K;
+  c->n = sum_n;
+  c->R = num_replicas;
+  c->M = 1;
+  c->prec = prec;
+  c->family = OJMI_KERNEL_BATCH_WAVE;
+  pick_types(c, maxabsJ);
+  if ((rc = common_init(c))) {
+    delete c;
+    return rc;
+  }
+  (void)hipFree(c->dE);
+  c->dE = nullptr;
+  if ((rc = dalloc(&c->dE, (size_t)(K * num_replicas)))) {
+    delete c;
+    return rc;
+  }
+  // layout: offsets, then one host image of couplings and fields
+  const size_t fsz = (c->ftype == FT_F64) ? 8 : 4;
+  std::vector<int64_t> j_off((size_t)K), h_off((size_t)K), s_off((size_t)K), rbv((size_t)K);
+  std::vector<int> NSk((size_t)K);
+  int64_t jbytes = 0, hcount = 0, scount = 0;
+  c->bn.resize((size_t)K);
+  for (int64_t k = 0; k < K; ++k) {
+    NSk[k] = pow2_slots(n[k]);
+    rbv[k] = batch_row_bytes(c->jtype, NSk[k]);
+    j_off[k] = jbytes;
+    jbytes += ((n[k] * rbv[k] + 255) / 256) * 256;
+    h_off[k] = hcount;
+    hcount += 64LL * NSk[k];
+    s_off[k] = scount;
+    scount += num_replicas * n[k];
+    c->bn[k] = (int32_t)n[k];
+  }
+  std::vector<char> Jimg((size_t)jbytes, 0), himg((size_t)hcount * fsz, 0);
+  for (int64_t k = 0; k < K; ++k) {
+    const double* Je = J_ext + ext_off[k];
+    const int64_t nk = n[k], n1 = nk + 1;
+    if (c->jtype == JT_I8) pack_rows<int8_t>(Je, nk, NSk[k], Jimg, (size_t)j_off[k]);
+    else if (c->jtype == JT_I32) pack_rows<int32_t>(Je, nk, NSk[k], Jimg, (size_t)j_off[k]);
+    else if (c->jtype == JT_F32) pack_rows<float>(Je, nk, NSk[k], Jimg, (size_t)j_off[k]);
+    else pack_rows<double>(Je, nk, NSk[k], Jimg, (size_t)j_off[k]);
+    for (int64_t i = 0; i < nk; ++i) {
+      const double hv = Je[i * n1 + nk];
+      char* dst = himg.data() + (size_t)(h_off[k] + i) * fsz;
+      if (c->ftype == FT_F64) std::memcpy(dst, &hv, 8);
+      else if (c->ftype == FT_F32) { const float x = (float)hv; std::memcpy(dst, &x, 4); }
+      else { const int32_t x = (int32_t)hv; std::memcpy(dst, &x, 4); }
+    }
+  }
+  auto up = [&](auto** d, const auto& v) -> int {
+    int r = dalloc(d, v.size());
+    if (r) return r;
+    OJMI_HIP(hipMemcpy(*d, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice));
+    return OJMI_OK;
+  };
+  char* dJc = nullptr;
+  char* dhc = nullptr;
+  rc = up(&dJc, Jimg);
+  c->dJ = dJc;  // owned by the context from here on
+  if (!rc) {
+    rc = up(&dhc, himg);
+    c->dh = dhc;
+  }
+  if (!rc) rc = up(&c->d_bj_off, j_off);
+  if (!rc) rc = up(&c->d_bh_off, h_off);
+  if (!rc) rc = up(&c->d_bs_off, s_off);
+  if (!rc) rc = up(&c->d_brb, rbv);
+  if (!rc) rc = up(&c->d_bn, c->bn);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  // one problem list per slot class, largest problems first within a class
+  for (int NS = 1; NS <= 64; NS <<= 1) {
+    std::vector<int32_t> lst;
+    for (int64_t k = 0; k < K; ++k)
+      if (NSk[k] == NS) lst.push_back((int32_t)k);
+    if (lst.empty()) continue;
+    std::stable_sort(lst.begin(), lst.end(), [&](int32_t a, int32_t b) { return n[a] > n[b]; });
+    int32_t* d = nullptr;
+    if ((rc = up(&d, lst))) {
+      delete c;
+      return rc;
+    }
+    c->bclass.push_back({NS, d});
+    c->bclass_count.push_back((int)lst.size());
+  }
+  *out = c;
+  return OJMI_OK;
+}
+
+int ojmi_run_batch(ojmi_ctx* c, const double* beta, const int64_t* mcs, int64_t L, uint64_t seed,
+                   uint64_t sweep_offset) {
+  if (!c) return fail(OJMI_EINVAL, "null context");
+  if (c->kind != 2) return fail(OJMI_EINVAL, "ojmi_run_batch needs a batch context");
+  if (L < 0 || (L > 0 && (!beta || !mcs))) return fail(OJMI_EINVAL, "bad schedule arrays");
+  int64_t T = 0;
+  for (int64_t l = 0; l < L; ++l) {
+    if (mcs[l] < 0) return fail(OJMI_EINVAL, "one_mc_step must be >= 0");
+    T += mcs[l];
+  }
+  for (int64_t q = 0; q < c->K * L; ++q)
+    if (!(beta[q] >= 0.0)) return fail(OJMI_EINVAL, "schedule beta range is '0 <= beta'.");
+  if (T > INT32_MAX || sweep_offset + (uint64_t)T > 0xffffffffull)
+    return fail(OJMI_EINVAL, "global sweep index exceeds 32 bits");
+  std::vector<double> p0((size_t)(c->K * T));
+  for (int64_t k = 0; k < c->K; ++k) {
+    int64_t t = 0;
+    for (int64_t l = 0; l < L; ++l) {
+      const double a0 = -1.0 / beta[k * L + l];
+      for (int64_t m = 0; m < mcs[l]; ++m, ++t) p0[(size_t)(k * T + t)] = a0;
+    }
+  }
+  return run_batch(c, p0, T, seed, sweep_offset);
+}
+
 int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   if (!c || !key) return fail(OJMI_EINVAL, "null argument");
   const std::string k(key);
@@ -657,6 +935,12 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
              uint64_t seed, uint64_t sweep_offset) {
   if (!c) return fail(OJMI_EINVAL, "null context");
   if (L < 0 || (L > 0 && (!beta || !mcs))) return fail(OJMI_EINVAL, "bad schedule arrays");
+  if (c->kind == 2) {  // one schedule for every problem of a batch
+    std::vector<double> rows((size_t)(c->K * L));
+    for (int64_t k = 0; k < c->K; ++k)
+      for (int64_t l = 0; l < L; ++l) rows[(size_t)(k * L + l)] = beta[l];
+    return ojmi_run_batch(c, rows.data(), mcs, L, seed, sweep_offset);
+  }
   const bool transverse = c->M > 1;
   if (transverse && L > 0 && !s) return fail(OJMI_EINVAL, "transverse system needs s[]");
   int64_t T = 0;
@@ -839,6 +1123,12 @@ static int compute_energies(ojmi_ctx* c) {
 
 int ojmi_energies(ojmi_ctx* c, double* out) {
   if (!c || !out) return fail(OJMI_EINVAL, "null argument");
+  if (c->kind == 2) {
+    int rc = batch_energies(c);
+    if (rc) return rc;
+    OJMI_HIP(hipMemcpy(out, c->dE, (size_t)(c->K * c->R) * 8, hipMemcpyDeviceToHost));
+    return OJMI_OK;
+  }
   int rc = compute_energies(c);
   if (rc) return rc;
   OJMI_HIP(hipMemcpy(out, c->dE, (size_t)(c->R * c->M) * 8, hipMemcpyDeviceToHost));
@@ -900,6 +1190,7 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->ring_rows = c->ring_rows_used;
   info->lds_bytes = c->lds_used;
   info->lds_budget = c->lds_budget;
+  info->num_problems = (c->kind == 2) ? c->K : 1;
   return OJMI_OK;
 }
 

@@ -27,6 +27,24 @@ struct DenseArgs {
   void* F_out;         // optional [R][Npad] final fields
 };
 
+// Batch of independent dense classical problems (k_dense.hip, k_batch_wave):
+// per-problem offsets into concatenated buffers; `inst` lists the problems of
+// one launch (one slot class), each annealed as R replicas.
+struct BatchArgs {
+  const void* J;         // couplings, problem k at byte offset j_off[k]
+  const void* h;         // fields' constant term, problem k at element h_off[k]
+  int8_t* spins;         // problem k: [R][n_k] at s_off[k]
+  const int64_t* j_off;
+  const int64_t* h_off;
+  const int64_t* s_off;
+  const int32_t* n;      // [K] spins per problem
+  const int32_t* inst;   // [num_items] problems of this launch
+  int num_items, R, T;
+  uint64_t seed, t0;
+  const double* p0_d;    // [K][T] -1/beta per sweep
+  const float* p0_f;
+};
+
 // CSR in level order (k_csr.hip): spins sorted by level, neighbours in ELL
 // form [D][n] indexed by level-order position.
 struct CsrArgs {
@@ -79,6 +97,9 @@ hipError_t launch_dense_colown(int ftype, int jtype, const SharedArgs& a, int NS
                                hipStream_t st);
 
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
+hipError_t launch_batch_wave(int ftype, int jtype, const BatchArgs& b, int NS, hipStream_t st);
+hipError_t launch_batch_energy(int ftype, int jtype, const BatchArgs& b, const int64_t* rb, int K,
+                               double* E, hipStream_t st);
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, hipStream_t st);
 

@@ -228,3 +228,81 @@ class SASampler(BaseSampler):
         resp.info["list_exec_times"] = exec_us
         resp.info["schedule"] = self.schedule_info
         return resp
+
+    def sample_batch(self, models, beta_min: Optional[float] = None,
+                     beta_max: Optional[float] = None, num_sweeps: Optional[int] = None,
+                     num_reads: Optional[int] = None, seed: Optional[int] = None,
+                     precision=None) -> list:
+        """Anneal independent models of different sizes in one launch.
+
+        Replaces a loop of ``sample`` calls, one per model (each a loop over
+        reads, openjij/sampler/sampler.py:115-140), by one batch context: every
+        model gets its own automatic geometric schedule (or the given bounds)
+        over the same number of sweeps, and ``num_reads`` replicas.  A BINARY
+        model with integer coefficients has quarter-integer Ising couplings;
+        they are scaled by 4 (and beta by 1/4, so beta * E is unchanged) to run
+        on the exact integer path.  Each model's random stream carries its
+        position in ``models``, so results differ from separate ``sample``
+        calls with the same seed.  Returns one Response per model."""
+        from openjij_amd import capi
+
+        self._set_params(beta_min=beta_min, beta_max=beta_max, num_sweeps=num_sweeps,
+                         num_reads=num_reads)
+        R = int(self._params["num_reads"])
+        sweeps = int(self._params["num_sweeps"])
+        seed_v = resolve_seed(seed)
+        models = list(models)
+        idx, J_exts, betas, scales, mcs = [], [], [], [], None
+        for k, bqm in enumerate(models):
+            if bqm.num_variables == 0:
+                continue
+            dense = BinaryQuadraticModel(bqm.linear, bqm.quadratic, bqm.offset, bqm.vartype,
+                                         sparse=False)
+            graph, _ = dense.get_cxxjij_ising_graph()
+            sched, _ = geometric_ising_beta_schedule(
+                graph, beta_max=self._params["beta_max"], beta_min=self._params["beta_min"],
+                num_sweeps=sweeps, seed=seed)
+            beta, m = schedule_arrays(sched)
+            h, J, _ = dense.ising_arrays()
+            n = dense.num_variables
+            ext = np.zeros((n + 1, n + 1))
+            ext[:n, :n] = J
+            ext[:n, n] = h
+            scale = 1.0
+            if not (np.all(ext == np.round(ext))) and np.all(4 * ext == np.round(4 * ext)):
+                scale = 4.0
+            ext *= scale
+            ext[n, n] = 1.0
+            idx.append(k)
+            J_exts.append(ext)
+            betas.append(beta / scale)
+            scales.append(scale)
+            mcs = m
+        t0 = time.perf_counter()
+        spins = {}
+        exec_us = 0.0
+        if idx:
+            ctx = capi.BatchContext.create(J_exts, R, _precision_code(precision))
+            try:
+                init = [initial_states(R, E.shape[0] - 1, seed_v, 0x5C + i)
+                        for i, E in enumerate(J_exts)]
+                ctx.set_spins(init)
+                ctx.run_batch(np.stack(betas), mcs, seed_v, 0)
+                out = ctx.solution()
+                exec_us = ctx.info()["last_kernel_ms"] * 1e3 / (R * len(idx))
+            finally:
+                ctx.close()
+            spins = dict(zip(idx, out))
+        sampling_us = (time.perf_counter() - t0) * 1e6
+        responses = []
+        for k, bqm in enumerate(models):
+            n = bqm.num_variables
+            sp = spins.get(k, np.zeros((R, 0), np.int8))
+            states = to_vartype_states(sp, bqm.vartype)
+            energies = bqm.energies(states) if n else np.full(R, bqm.offset)
+            resp = Response(states, energies, bqm.variables, bqm.vartype, {})
+            resp.info["sampling_time"] = sampling_us
+            resp.info["execution_time"] = exec_us
+            resp.info["list_exec_times"] = np.full(R, exec_us)
+            responses.append(resp)
+        return responses

@@ -548,7 +548,7 @@ int orc_brute_force_dense(i64 n, const double* J, double* emin, i8* best) {
 }  // extern "C"
 template <typename F>
 static void replay_dense_T(int mode, i64 n, const double* J, i64 R, i8* spins, u64 seed,
-                           u64 t0, const double* nbinv, i64 T) {
+                           u64 t0, const double* nbinv, i64 T, u32 word3) {
   const i64 n1 = n + 1;
   std::vector<F> Jc((size_t)n * n), h(n);
   for (i64 i = 0; i < n; ++i) {
@@ -573,7 +573,7 @@ static void replay_dense_T(int mode, i64 n, const double* J, i64 R, i8* spins, u
       const double nb = nbinv[t];
       const float nbf = (float)nb;
       for (i64 i = 0; i < n; ++i) {
-        const u32 u = philox_word(seed, tg, r, 0, i);
+        const u32 u = philox_word(seed, tg, r, word3, i);
         const F dE = (F)(-2 * s[i]) * Fv[i];
         bool acc;
         if (mode == 1) acc = (float)dE < theta_f(u, nbf);
@@ -590,16 +590,25 @@ static void replay_dense_T(int mode, i64 n, const double* J, i64 R, i8* spins, u
 }
 extern "C" {
 
-int orc_replay_dense(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64 seed,
-                     u64 sweep_offset, const double* nbinv, i64 T) {
+// word3: the fourth Philox counter word (0 for a single problem; the problem
+// index inside a batch of problems).
+int orc_replay_dense_w(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64 seed,
+                       u64 sweep_offset, const double* nbinv, i64 T, u32 word3) {
   if (n < 1 || R < 0 || T < 0) return 1;
-  if (mode == 0) replay_dense_T<double>(0, n, J_ext, R, spins, seed, sweep_offset, nbinv, T);
-  else if (mode == 1) replay_dense_T<float>(1, n, J_ext, R, spins, seed, sweep_offset, nbinv, T);
+  if (mode == 0)
+    replay_dense_T<double>(0, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3);
+  else if (mode == 1)
+    replay_dense_T<float>(1, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3);
   else if (mode == 2) {
     if (!is_integer_matrix(J_ext, (n + 1) * (n + 1))) return 2;
-    replay_dense_T<int32_t>(2, n, J_ext, R, spins, seed, sweep_offset, nbinv, T);
+    replay_dense_T<int32_t>(2, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3);
   } else return 1;
   return 0;
+}
+
+int orc_replay_dense(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64 seed,
+                     u64 sweep_offset, const double* nbinv, i64 T) {
+  return orc_replay_dense_w(mode, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, 0);
 }
 
 // CSR replay: the field of spin i is recomputed at each attempt in stored

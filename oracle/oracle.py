@@ -49,6 +49,7 @@ def lib():
             "orc_energy_exact_csr": (dbl, [i64, vp, vp, vp, vp]),
             "orc_brute_force_dense": (C.c_int, [i64, vp, vp, vp]),
             "orc_replay_dense": (C.c_int, [C.c_int, i64, vp, i64, vp, u64, u64, vp, i64]),
+            "orc_replay_dense_w": (C.c_int, [C.c_int, i64, vp, i64, vp, u64, u64, vp, i64, u32]),
             "orc_replay_csr": (C.c_int, [C.c_int, i64, vp, vp, vp, i64, vp, u64, u64, vp, i64]),
             "orc_replay_sqa_dense": (C.c_int, [C.c_int, i64, vp, i64, i64, vp, u64, u64, vp, vp, i64]),
             "orc_replay_sqa_csr": (C.c_int, [C.c_int, i64, vp, vp, vp, i64, i64, vp, u64, u64, vp, vp, i64]),
@@ -207,12 +208,13 @@ def brute_force(J_ext):
     return e.value, st
 
 
-def replay_dense(mode, J_ext, spins, seed, sweep_offset, nbinv):
+def replay_dense(mode, J_ext, spins, seed, sweep_offset, nbinv, word3=0):
+    """word3: fourth Philox counter word (the problem index inside a batch)."""
     J = _f64(J_ext)
     s = _i8(spins).copy()
     nb = _f64(nbinv)
-    rc = lib().orc_replay_dense(MODE[mode], J.shape[0] - 1, _p(J), s.shape[0], _p(s), seed,
-                                sweep_offset, _p(nb), len(nb))
+    rc = lib().orc_replay_dense_w(MODE[mode], J.shape[0] - 1, _p(J), s.shape[0], _p(s), seed,
+                                  sweep_offset, _p(nb), len(nb), word3)
     assert rc == 0, rc
     return s
 

@@ -123,3 +123,33 @@ def ks_2samp_pvalue(a, b) -> float:
     from scipy import stats
 
     return float(stats.ks_2samp(a, b).pvalue)
+
+
+def qubo_batch_sizes(K: int, seed: int) -> np.ndarray:
+    """N_k = round(32 * 32**u), u uniform on [0, 1) from splitmix64: log-uniform
+    sizes in [32, 1024]."""
+    w = splitmix64_np(seed, K)
+    u = (w >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return np.rint(32.0 * 32.0 ** u).astype(np.int64)
+
+
+def qubo_upper(n: int, seed: int, lo: int = -8, hi: int = 8, density: float = 0.5) -> np.ndarray:
+    """Integer QUBO, entries Q_ij for i <= j present with probability `density`,
+    uniform in [lo, hi]; E(x) = sum_{i<=j} Q_ij x_i x_j."""
+    w = splitmix64_np(seed, 2 * n * n)
+    span = np.uint64(hi - lo + 1)
+    vals = ((w[: n * n] >> np.uint64(32)) * span >> np.uint64(32)).astype(np.int64) + lo
+    keep = (w[n * n:] >> np.uint64(11)).astype(np.float64) * 2.0 ** -53 < density
+    return np.triu(np.where(keep, vals, 0).reshape(n, n))
+
+
+def qubo_to_scaled_ising(Q: np.ndarray):
+    """4 x the Ising form of E(x) = sum_{i<=j} Q_ij x_i x_j under x = (1+s)/2:
+    J'_ij = Q_ij (i<j), h'_i = 2 Q_ii + sum_{j != i} Q_{min(i,j),max(i,j)},
+    so 4 E(x) = E'(s) + C with C = sum_{i<j} Q_ij + 2 sum_i Q_ii.  Integer Q
+    stays integer.  Returns (J_ext of E', C)."""
+    n = Q.shape[0]
+    U = np.triu(Q, 1).astype(float)
+    d = np.diag(Q).astype(float)
+    h = 2.0 * d + U.sum(axis=1) + U.sum(axis=0)
+    return ext_from_upper(U, h), float(U.sum() + 2.0 * d.sum())

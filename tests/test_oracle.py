@@ -294,3 +294,21 @@ def test_replay_sqa_modes_and_edges():
     # real couplings: recompute vs incremental may differ in the last bit, but
     # the final energies agree statistically; here just well-formed +-1
     assert set(np.unique(out2)) <= {-1, 1}
+
+
+def test_replay_fourth_counter_word_selects_the_stream():
+    """One spin with field h: each sweep flips it iff -2 s h < theta(u) with
+    u = philox_word(seed, t, r=0, m=word3, i=0); the replay must follow the
+    word it is given (the problem index of a batch)."""
+    h, seed, T = 0.7, 0xBEEF, 40
+    J = np.array([[0.0, h], [h, 1.0]])
+    nb = -1.0 / np.linspace(0.2, 3.0, T)
+    for word3 in (0, 5, 4095):
+        s = 1
+        for t in range(T):
+            u = orc.philox_word(seed, t, 0, word3, 0)
+            U = (u + 0.5) * 2.0 ** -32  # the FP64 uniform of the threshold rule
+            if (-2.0 * s * h) < orc.det_log_d(U) * nb[t]:
+                s = -s
+        got = orc.replay_dense("fp64", J, np.array([[1]], np.int8), seed, 0, nb, word3=word3)
+        assert got[0, 0] == s

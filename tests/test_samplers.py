@@ -296,3 +296,26 @@ def test_sa_random_models_reach_brute_force_minimum(n, seed):
     bqm = oj.BinaryQuadraticModel.from_ising(h, J)
     res = oj.SASampler().sample_ising(h, J, num_reads=32, num_sweeps=2000, seed=seed)
     assert res.first.energy == pytest.approx(brute_min(bqm), abs=1e-9)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [None, "fp64"])
+def test_sample_batch_reaches_every_minimum(precision):
+    rng = np.random.default_rng(12)
+    models = []
+    for n in (3, 9, 14, 6, 12):
+        Q = {(i, j): int(rng.integers(-8, 9)) for i in range(n) for j in range(i, n)
+             if rng.random() < 0.5}
+        models.append(oj.BinaryQuadraticModel.from_qubo(Q))
+    models.insert(2, oj.BinaryQuadraticModel.from_qubo({}))
+    models.append(oj.BinaryQuadraticModel.from_ising({0: 0.5}, {(0, 1): -1.25, (1, 2): 0.75}))
+    res = oj.SASampler().sample_batch(models, num_reads=16, num_sweeps=500, seed=3,
+                                      precision=precision)
+    assert len(res) == len(models)
+    for bqm, r in zip(models, res):
+        assert len(r.states) == 16
+        if bqm.num_variables == 0:
+            assert r.states.shape == (16, 0)
+            continue
+        np.testing.assert_allclose(r.energies, bqm.energies(r.states), atol=1e-9)
+        assert r.first.energy == pytest.approx(brute_min(bqm), abs=1e-9)
