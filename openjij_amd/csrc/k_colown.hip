@@ -20,6 +20,8 @@
 #include "ojmi_kernels.h"
 #include "ojmi_wave.h"
 
+#include <type_traits>
+
 namespace ojmi {
 
 template <typename FT>
@@ -33,27 +35,41 @@ struct ColThresh<float> {
 
 namespace {
 
-template <int BYTES>
-__device__ __forceinline__ void g2lds_c(const char* gsrc, char* ldst_wave) {
-  auto* l = (__attribute__((address_space(3))) void*)ldst_wave;
-  if constexpr (BYTES == 16) {
-    __builtin_amdgcn_global_load_lds((const void*)gsrc, l, 16, 0, 0);
-  } else {
-    static_assert(BYTES == 4, "lane width must be 4 or 16 bytes");
-    __builtin_amdgcn_global_load_lds((const void*)gsrc, l, 4, 0, 0);
+// Diagonal tile D[i][l] = J[64k+i][64k+l] (natural-layout rows of Npad),
+// staged through registers: wave w loads rows 4q..4q+3 for q = w, w+G, ...,
+// four consecutive columns per lane, and later stores them to LDS in the same
+// order.  Plain vector loads keep the count of loads in flight exact, so row
+// loads issued after them can be waited for individually.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <typename JT>
+struct TileVec {
+  typedef typename std::conditional<sizeof(JT) == 1, uint32_t, u32x4>::type type;
+};
+
+template <typename JT, int NS, int G>
+__device__ __forceinline__ void load_tile(typename TileVec<JT>::type (&st)[16 / G],
+                                          const char* __restrict__ J, int k, int w, int lane) {
+  constexpr int EB = (int)sizeof(JT);
+  constexpr int64_t RB = (int64_t)64 * NS * EB;
+  typedef typename TileVec<JT>::type V;
+#pragma unroll
+  for (int j = 0; j < 16 / G; ++j) {
+    const int q = w + G * j;
+    const int i = 4 * q + (lane >> 4);
+    st[j] = *reinterpret_cast<const V*>(J + (int64_t)(64 * k + i) * RB +
+                                         (64 * k + 4 * (lane & 15)) * EB);
   }
 }
 
-// Diagonal tile D[i][l] = J[64k+i][64k+l], natural-layout rows of Npad.
-template <typename JT, int NS, int G>
-__device__ __forceinline__ void issue_tile_c(char* dst, const char* __restrict__ J, int k, int w,
-                                             int lane) {
+template <typename JT, int G>
+__device__ __forceinline__ void store_tile(const typename TileVec<JT>::type (&st)[16 / G],
+                                           char* dst, int w, int lane) {
   constexpr int EB = (int)sizeof(JT);
-  constexpr int64_t RB = (int64_t)64 * NS * EB;
-  for (int q = w; q < 16; q += G) {
-    const int i = 4 * q + (lane >> 4);
-    const char* src = J + (int64_t)(64 * k + i) * RB + (64 * k + 4 * (lane & 15)) * EB;
-    g2lds_c<4 * EB>(src, dst + q * (4 * 64 * EB));
+  typedef typename TileVec<JT>::type V;
+#pragma unroll
+  for (int j = 0; j < 16 / G; ++j) {
+    const int q = w + G * j;
+    *reinterpret_cast<V*>(dst + q * (4 * 64 * EB) + lane * (4 * EB)) = st[j];
   }
 }
 
@@ -79,6 +95,38 @@ __device__ __forceinline__ uint32_t peer_min(const uint32_t* prog, int b, int nb
   return mn;
 }
 
+// A wave-uniform 64-bit value read from LDS, moved to scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// w[j] for a wave-uniform j, without indexing a register array (which the
+// compiler would place in scratch memory when the block loop is not unrolled).
+__device__ __forceinline__ uint32_t pick4(const uint32_t (&w)[4], int j) {
+  return j == 0 ? w[0] : (j == 1 ? w[1] : (j == 2 ? w[2] : w[3]));
+}
+
+// s_waitcnt: at most `vm` vector memory loads still in flight (gfx9 encoding:
+// vmcnt in bits 3:0 and 15:14, expcnt and lgkmcnt left at "no wait").
+template <int vm>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(vm >= 0 && vm < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((vm & 15) | (((vm >> 4) & 3) << 14) | (7 << 4) | (15 << 8));
+}
+
+// +2 or -2 as a wave-uniform (scalar register) value.
+template <typename FT>
+__device__ __forceinline__ FT coef2(bool plus) {
+  if constexpr (sizeof(FT) == 4 && (FT)0.5 != (FT)0) {  // float
+    const uint32_t b = __builtin_amdgcn_readfirstlane(plus ? 0x40000000u : 0xC0000000u);
+    return __builtin_bit_cast(FT, b);
+  } else {
+    return (FT)(int)__builtin_amdgcn_readfirstlane(plus ? 2 : -2);
+  }
+}
+
 // x[s] = element (chunk slot s, lane) of one coupling row
 template <typename JT, typename FT, int CS>
 __device__ __forceinline__ void load_chunk(FT (&x)[CS], const JT* __restrict__ p, int lane) {
@@ -98,8 +146,8 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
   constexpr int PF = (CS >= 8) ? 2 : ((CS >= 4) ? 4 : 8);  // rows per prefetch group
 
   extern __shared__ __align__(16) char smem[];
-  char* sD = smem;                                                   // 2 tiles
-  FT* sF = reinterpret_cast<FT*>(smem + 2 * TILE_BYTES);             // [G][64]
+  char* sD = smem;                                                   // 1 tile
+  FT* sF = reinterpret_cast<FT*>(smem + TILE_BYTES);                 // [G][64]
   uint64_t* sS = reinterpret_cast<uint64_t*>(sF + G * 64);           // [G]
   uint64_t* sFlip = sS + G;                                          // [G]
   uint64_t* sSign = sFlip + G;                                       // [G]
@@ -180,11 +228,11 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
   constexpr int NQ = (NS + 3) / 4;
   const int nblocks = gridDim.x;
   uint32_t done = 0;
-  int tb = 0;
+  typename TileVec<JT>::type tile[16 / G];
 
   if (a.T > 0) {
-    issue_tile_c<JT, NS, G>(sD, reinterpret_cast<const char*>(J), 0, w, lane);
-    __builtin_amdgcn_s_waitcnt(0);
+    load_tile<JT, NS, G>(tile, reinterpret_cast<const char*>(J), 0, w, lane);
+    store_tile<JT, G>(tile, sD, w, lane);  // visible after the first barrier
   }
 
   for (int t = 0; t < a.T; ++t) {
@@ -217,26 +265,27 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
         // ---- A: wave w decides block k for replica r
         uint64_t flips = 0, sb0 = 0;
         if (active) {
-          const JT* D = reinterpret_cast<const JT*>(sD + tb * TILE_BYTES);
+          const JT* D = reinterpret_cast<const JT*>(sD);
           FieldVec<FT, 64> Dc;  // Dc[i] = D[i][lane]
 #pragma unroll
           for (int q = 0; q < 64; ++q) Dc.set(q, (FT)D[q * 64 + lane]);
           __builtin_amdgcn_sched_barrier(0);
           const int i = 64 * k + lane;
-          const TT th = (i < n) ? log_uniform<TT>(wd[jj]) * nb : neg_inf<TT>();
+          const TT th = (i < n) ? log_uniform<TT>(pick4(wd, jj)) * nb : neg_inf<TT>();
           uint64_t live = ballot64(i < n);
-          uint64_t sb = sS[w];
+          // A lane is never tested again once it has flipped, so within the
+          // block every live lane keeps its sign: dE = s2 * F.
+          const uint64_t sb = uniform64(sS[w]);
           sb0 = sb;
+          const FT s2 = ((sb >> lane) & 1ull) ? (FT)2 : (FT)(-2);
           FT fk = sF[w * 64 + lane];
           while (true) {
-            const bool dn = (sb >> lane) & 1ull;
-            const FT dE = dn ? (FT)2 * fk : (FT)(-2) * fk;
+            const FT dE = s2 * fk;
             const uint64_t cand = ballot64((TT)dE < th) & live;
             if (cand == 0) break;
             const int i0 = __builtin_ctzll(cand);
             live &= lanes_above(i0);
             const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
-            sb ^= 1ull << i0;
             flips |= 1ull << i0;
             fk = fma_ft<FT>(c, Dc.dyn(i0), fk);
           }
@@ -250,8 +299,8 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
         uint64_t U = 0;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          fl[g] = sFlip[g];
-          sg[g] = sSign[g];
+          fl[g] = uniform64(sFlip[g]);
+          sg[g] = uniform64(sSign[g]);
           U |= fl[g];
         }
         if (w == ow) {
@@ -266,21 +315,24 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
             __builtin_amdgcn_s_sleep(4);
           }
         }
-        // next block's tile into the other buffer
+        // next block's tile: loaded now, stored to LDS after the row updates
+        // (every wave has its decisions' tile column in registers by now)
         const int kn = (k + 1 < NS) ? k + 1 : 0;
-        issue_tile_c<JT, NS, G>(sD + (tb ^ 1) * TILE_BYTES, reinterpret_cast<const char*>(J), kn,
-                                w, lane);
-        // ---- B: union rows, ascending, PF rows per load group, two groups in flight
+        load_tile<JT, NS, G>(tile, reinterpret_cast<const char*>(J), kn, w, lane);
+        // ---- B: union rows, ascending, PF rows per load group, two groups in
+        // flight.  Every group issues exactly PF row loads (a drained mask loads
+        // row 0 of the block as a dummy), so the number of loads in flight is
+        // the same on every path and each apply waits only for its own group.
         const JT* Jk = J + (int64_t)(64 * k) * Npad + col0;
         uint64_t Ug = U;  // rows not yet loaded
-        uint64_t Uc = U;  // rows not yet applied
+        const int ng = (__builtin_popcountll(U) + PF - 1) / PF;
         FT xa[PF][CS], xb[PF][CS];
         int ua[PF], ub[PF];
         auto load_group = [&](FT (&x)[PF][CS], int (&uu)[PF]) {
 #pragma unroll
           for (int p = 0; p < PF; ++p) {
             const int u = Ug ? __builtin_ctzll(Ug) : -1;
-            if (Ug) Ug &= Ug - 1;
+            Ug &= Ug - 1;
             uu[p] = u;
             load_chunk<JT, FT, CS>(x[p], Jk + (int64_t)(u < 0 ? 0 : u) * Npad, lane);
           }
@@ -293,7 +345,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
               if ((fl[g] >> u) & 1ull) {
-                const FT c = ((sg[g] >> u) & 1ull) ? (FT)2 : (FT)(-2);
+                const FT c = coef2<FT>((sg[g] >> u) & 1ull);
 #pragma unroll
                 for (int s = 0; s < CS; ++s)
                   F.set(g * CS + s, fma_ft<FT>(c, x[p][s], F.get(g * CS + s)));
@@ -301,23 +353,34 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
             }
           }
         };
-        if (Uc) load_group(xa, ua);
-        while (Uc) {
-          if (Ug) load_group(xb, ub);
-          __builtin_amdgcn_sched_barrier(0);
-          apply_group(xa, ua);
-#pragma unroll
-          for (int p = 0; p < PF && Uc; ++p) Uc &= Uc - 1;
-          if (!Uc) break;
-          if (Ug) load_group(xa, ua);
-          __builtin_amdgcn_sched_barrier(0);
-          apply_group(xb, ub);
-#pragma unroll
-          for (int p = 0; p < PF && Uc; ++p) Uc &= Uc - 1;
+        if (ng > 0) {
+          load_group(xa, ua);
+          int j = 0;
+          // each apply waits once for its own group (the other group's
+          // PF * CS loads may stay in flight)
+          for (; j + 2 < ng; j += 2) {
+            load_group(xb, ub);
+            __builtin_amdgcn_sched_barrier(0);
+            wait_vm<PF * CS>();
+            apply_group(xa, ua);
+            load_group(xa, ua);
+            __builtin_amdgcn_sched_barrier(0);
+            wait_vm<PF * CS>();
+            apply_group(xb, ub);
+          }
+          if (j + 1 < ng) {
+            load_group(xb, ub);
+            __builtin_amdgcn_sched_barrier(0);
+            wait_vm<PF * CS>();
+            apply_group(xa, ua);
+            wait_vm<0>();
+            apply_group(xb, ub);
+          } else {
+            wait_vm<0>();
+            apply_group(xa, ua);
+          }
         }
-        // the next tile must have landed before the next block's decisions
-        __builtin_amdgcn_s_waitcnt(0);
-        tb ^= 1;
+        store_tile<JT, G>(tile, sD, w, lane);
         ++done;
         if (a.xcd_window > 0 && tid == 0)
           __hip_atomic_store(a.progress + b, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -344,7 +407,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
 
 size_t colown_lds_bytes(int n, int NS, int G, int jbytes, int fbytes) {
   (void)NS;
-  return (size_t)2 * 64 * 64 * jbytes + (size_t)G * 64 * fbytes + 3 * 8 * (size_t)G +
+  return (size_t)64 * 64 * jbytes + (size_t)G * 64 * fbytes + 3 * 8 * (size_t)G +
          4 * (size_t)((n + 3) / 4) + 16;
 }
 

@@ -102,20 +102,23 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
         const int i = 64 * k + lane;
         const TT th = (i < n) ? log_uniform<TT>(w[jj]) * nb : neg_inf<TT>();
         uint64_t live = ballot64(i < n);
-        uint64_t sb = ballot64((down >> k) & 1ull);
+        // a lane is never tested again once it has flipped, so every live
+        // lane keeps its sign within the slot: dE = s2 * F
+        const uint64_t sb = ballot64((down >> k) & 1ull);
+        const FT s2 = ((down >> k) & 1ull) ? (FT)2 : (FT)(-2);
+        uint64_t flips = 0;
         while (true) {
           const FT fk = F.dyn(k);
-          const bool dn = (sb >> lane) & 1ull;
-          const FT dE = dn ? (FT)2 * fk : (FT)(-2) * fk;
+          const FT dE = s2 * fk;
           const uint64_t cand = ballot64((TT)dE < th) & live;
           if (cand == 0) break;
           const int i0 = __builtin_ctzll(cand);
           live &= lanes_above(i0);
           const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
-          sb ^= 1ull << i0;
-          if (lane == i0) down ^= 1ull << k;
+          flips |= 1ull << i0;
           row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c);
         }
+        down ^= ((flips >> lane) & 1ull) << k;
       }
     }
   }
