@@ -254,21 +254,20 @@ def tts99(energies, e_ref, tol, tau):
 
 
 def profiled_traffic(kernel_name, workload_name):
-    """HBM bytes per dispatch of `kernel_name` on `workload_name`, from the
-    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE, scaled by the
-    copy-kernel calibration).  None when no summary covers this workload."""
-    path = os.path.join(ROOT, "profiles", "dense_colown_sk4096_pmc.txt")
-    try:
-        text = open(path).read()
-    except OSError:
-        return None
-    kv = dict(re.findall(r"^([a-z_]+):\s*(.+?)\s*$", text, re.M))
-    if kv.get("kernel") != kernel_name or kv.get("workload") != workload_name:
-        return None
-    try:
-        return float(kv["hbm_bytes_per_dispatch"])
-    except (KeyError, ValueError):
-        return None
+    """HBM bytes per run of `kernel_name` on `workload_name` (all of the run's
+    launches), from a committed rocprofv3 counter summary under profiles/
+    (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration).  None
+    when no summary covers this kernel and workload."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_pmc.txt"))):
+        kv = dict(re.findall(r"^([a-z_]+):\s*(.+?)\s*$", open(path).read(), re.M))
+        if kv.get("kernel") == kernel_name and kv.get("workload") == workload_name:
+            try:
+                return float(kv["hbm_bytes_per_run"])
+            except (KeyError, ValueError):
+                return None
+    return None
 
 
 def calibrate_copy():

@@ -119,7 +119,7 @@ __device__ __forceinline__ void wait_vm() {
 // +2 or -2 as a wave-uniform (scalar register) value.
 template <typename FT>
 __device__ __forceinline__ FT coef2(bool plus) {
-  if constexpr (sizeof(FT) == 4 && (FT)0.5 != (FT)0) {  // float
+  if constexpr (std::is_floating_point<FT>::value) {
     const uint32_t b = __builtin_amdgcn_readfirstlane(plus ? 0x40000000u : 0xC0000000u);
     return __builtin_bit_cast(FT, b);
   } else {

@@ -164,11 +164,24 @@ __global__ __launch_bounds__(256) void k_dense_wave(DenseArgs a) {
 // wave g anneals replica g % R of problem inst[g / R], whose couplings,
 // fields, spins and per-sweep schedule start at its own offsets.  The problem
 // index is the fourth word of the random counter.
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8).  This bijection of [0, nb) keeps runs of `chunk` consecutive
+// logical workgroups -- the replicas of one problem -- on one XCD, so they
+// share its L2, while consecutive runs still rotate over the XCDs (problems of
+// similar size spread evenly).  The last incomplete round keeps b.
+__device__ __forceinline__ int xcd_chunked(int b, int nb, int chunk) {
+  const int round = 8 * chunk;
+  if (b >= (nb / round) * round) return b;
+  const int xcd = b & 7, idx = b >> 3;
+  return (idx / chunk) * round + xcd * chunk + (idx % chunk);
+}
+
 template <typename FT, typename JT, int NS>
 __global__ __launch_bounds__(256) void k_batch_wave(BatchArgs b) {
   typedef typename ThreshT<FT>::type TT;
   const int lane = threadIdx.x & 63;
-  const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+  const int wg = xcd_chunked(blockIdx.x, gridDim.x, (b.R + 3) >> 2);
+  const int g = __builtin_amdgcn_readfirstlane(wg * (blockDim.x >> 6) + (threadIdx.x >> 6));
   if (g >= b.num_items * b.R) return;
   const int k = b.inst[g / b.R];
   const int r = g - (g / b.R) * b.R;
