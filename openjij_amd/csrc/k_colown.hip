@@ -127,11 +127,42 @@ __device__ __forceinline__ FT coef2(bool plus) {
   }
 }
 
-// x[s] = element (chunk slot s, lane) of one coupling row
+// Column layout of a wave's chunk: each lane loads VEC consecutive columns
+// (16 bytes where the chunk allows), so slot s of lane l is column
+//   col0 + 64*VEC*(s / VEC) + VEC*l + s % VEC.
+template <typename JT, int CS>
+__host__ __device__ constexpr int vec_cols() {
+  return (int)(16 / sizeof(JT)) < CS ? (int)(16 / sizeof(JT)) : CS;
+}
+template <typename JT, int V>
+struct JVec {
+  typedef JT type __attribute__((ext_vector_type(V)));
+};
+template <typename JT>
+struct JVec<JT, 1> {
+  typedef JT type;
+};
+
+template <int VEC>
+__device__ __forceinline__ int slot_col(int col0, int s, int lane) {
+  return col0 + 64 * VEC * (s / VEC) + VEC * lane + (s % VEC);
+}
+
+// x[s] = element (slot s, lane) of one coupling row chunk starting at p
 template <typename JT, typename FT, int CS>
 __device__ __forceinline__ void load_chunk(FT (&x)[CS], const JT* __restrict__ p, int lane) {
+  constexpr int VEC = vec_cols<JT, CS>();
+  typedef typename JVec<JT, VEC>::type V;
 #pragma unroll
-  for (int s = 0; s < CS; ++s) x[s] = (FT)p[64 * s + lane];
+  for (int q = 0; q < CS / VEC; ++q) {
+    const V v = *reinterpret_cast<const V*>(p + 64 * VEC * q + VEC * lane);
+    if constexpr (VEC == 1) {
+      x[q] = (FT)v;
+    } else {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) x[VEC * q + e] = (FT)v[e];
+    }
+  }
 }
 
 }  // namespace
@@ -141,6 +172,8 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
   typedef typename ColThresh<FT>::type TT;
   constexpr int CS = NS / G;
   static_assert(CS >= 1 && CS * G == NS, "NS must be a multiple of G");
+  constexpr int VEC = vec_cols<JT, CS>();  // columns per lane per load
+  static_assert(CS % VEC == 0 && 64 % VEC == 0, "chunk layout");
   constexpr int Npad = 64 * NS;
   constexpr int TILE_BYTES = 64 * 64 * (int)sizeof(JT);
   constexpr int PF = (CS >= 8) ? 2 : ((CS >= 4) ? 4 : 8);  // rows per prefetch group
@@ -148,10 +181,10 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
   extern __shared__ __align__(16) char smem[];
   char* sD = smem;                                                   // 1 tile
   FT* sF = reinterpret_cast<FT*>(smem + TILE_BYTES);                 // [G][64]
-  uint64_t* sS = reinterpret_cast<uint64_t*>(sF + G * 64);           // [G]
-  uint64_t* sFlip = sS + G;                                          // [G]
+  uint64_t* sFlip = reinterpret_cast<uint64_t*>(sF + G * 64);        // [G]
   uint64_t* sSign = sFlip + G;                                       // [G]
-  uint32_t* sTab = reinterpret_cast<uint32_t*>(sSign + G);           // [(n+3)/4]
+  uint8_t* sDn = reinterpret_cast<uint8_t*>(sSign + G);              // [G][64]
+  uint32_t* sTab = reinterpret_cast<uint32_t*>(sDn + G * 64);        // [(n+3)/4]
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -170,7 +203,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
       const int8_t* sp = a.spins + (int64_t)(rbase + g) * n;
 #pragma unroll
       for (int s = 0; s < CS; ++s) {
-        const int col = col0 + 64 * s + lane;
+        const int col = slot_col<VEC>(col0, s, lane);
         if (col < n && sp[col] < 0) down |= 1ull << (g * CS + s);
       }
     }
@@ -183,7 +216,8 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     for (int g = 0; g < G; ++g) {
       const int rg = (rbase + g < a.R) ? rbase + g : 0;
 #pragma unroll
-      for (int s = 0; s < CS; ++s) F.set(g * CS + s, fi[(int64_t)rg * Npad + col0 + 64 * s + lane]);
+      for (int s = 0; s < CS; ++s)
+        F.set(g * CS + s, fi[(int64_t)rg * Npad + slot_col<VEC>(col0, s, lane)]);
     }
   } else {
     // F = h + sum_k s_k J[k][.], k in order, for every replica; the spins of
@@ -195,7 +229,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     for (int g = 0; g < G; ++g)
 #pragma unroll
       for (int s = 0; s < CS; ++s) {
-        const int col = col0 + 64 * s + lane;
+        const int col = slot_col<VEC>(col0, s, lane);
         if ((down >> (g * CS + s)) & 1ull)
           atomicOr(&sTab[col >> 2], 1u << ((col & 3) * 8 + g));
       }
@@ -203,7 +237,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     const FT* h = reinterpret_cast<const FT*>(a.h);
 #pragma unroll
     for (int s = 0; s < CS; ++s) {
-      const FT hv = h[col0 + 64 * s + lane];
+      const FT hv = h[slot_col<VEC>(col0, s, lane)];
 #pragma unroll
       for (int g = 0; g < G; ++g) F.set(g * CS + s, hv);
     }
@@ -252,13 +286,21 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
         const int k = 4 * Q + jj;
         if (k >= NS) break;
         const int ow = k / CS, sk = k - ow * CS;
-        // ---- 0: the owner publishes block k's fields and spin words
-        if (w == ow) {
+        // block k = columns 64*sk + t of the owner's chunk: slots VEC*qb + e
+        // of lanes lb .. lb + 64/VEC - 1, column t = VEC*(lane - lb) + e
+        const int qb = sk / VEC;
+        const int lb = (64 / VEC) * (sk % VEC);
+        const bool holds = lane >= lb && lane < lb + 64 / VEC;
+        // ---- 0: the owner publishes block k's fields and spin states
+        if (w == ow && holds) {
 #pragma unroll
           for (int g = 0; g < G; ++g) {
-            sF[g * 64 + lane] = F.dyn(g * CS + sk);
-            const uint64_t bal = ballot64((down >> (g * CS + sk)) & 1ull);
-            if (lane == 0) sS[g] = bal;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+              const int t = VEC * (lane - lb) + e;
+              sF[g * 64 + t] = F.dyn(g * CS + VEC * qb + e);
+              sDn[g * 64 + t] = (uint8_t)((down >> (g * CS + VEC * qb + e)) & 1ull);
+            }
           }
         }
         bar();
@@ -275,9 +317,10 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
           uint64_t live = ballot64(i < n);
           // A lane is never tested again once it has flipped, so within the
           // block every live lane keeps its sign: dE = s2 * F.
-          const uint64_t sb = uniform64(sS[w]);
+          const bool dn = sDn[w * 64 + lane] != 0;
+          const uint64_t sb = ballot64(dn);
           sb0 = sb;
-          const FT s2 = ((sb >> lane) & 1ull) ? (FT)2 : (FT)(-2);
+          const FT s2 = dn ? (FT)2 : (FT)(-2);
           FT fk = sF[w * 64 + lane];
           while (true) {
             const FT dE = s2 * fk;
@@ -303,9 +346,11 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
           sg[g] = uniform64(sSign[g]);
           U |= fl[g];
         }
-        if (w == ow) {
+        if (w == ow && holds) {
+          constexpr uint64_t m = (VEC == 64) ? ~0ull : ((1ull << VEC) - 1);
 #pragma unroll
-          for (int g = 0; g < G; ++g) down ^= ((fl[g] >> lane) & 1ull) << (g * CS + sk);
+          for (int g = 0; g < G; ++g)
+            down ^= ((fl[g] >> (VEC * (lane - lb))) & m) << (g * CS + VEC * qb);
         }
         // soft XCD pacing (performance only, bounded)
         if (a.xcd_window > 0 && w == 0 && done > (uint32_t)a.xcd_window) {
@@ -394,21 +439,23 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     int8_t* sp = a.spins + (int64_t)(rbase + g) * n;
 #pragma unroll
     for (int s = 0; s < CS; ++s) {
-      const int col = col0 + 64 * s + lane;
+      const int col = slot_col<VEC>(col0, s, lane);
       if (col < n) sp[col] = ((down >> (g * CS + s)) & 1ull) ? (int8_t)-1 : (int8_t)1;
     }
     if (a.F_out) {
       FT* fo = reinterpret_cast<FT*>(a.F_out) + (int64_t)(rbase + g) * Npad;
 #pragma unroll
-      for (int s = 0; s < CS; ++s) fo[col0 + 64 * s + lane] = F.get(g * CS + s);
+      for (int s = 0; s < CS; ++s) fo[slot_col<VEC>(col0, s, lane)] = F.get(g * CS + s);
     }
   }
 }
 
 size_t colown_lds_bytes(int n, int NS, int G, int jbytes, int fbytes) {
   (void)NS;
-  return (size_t)64 * 64 * jbytes + (size_t)G * 64 * fbytes + 3 * 8 * (size_t)G +
-         4 * (size_t)((n + 3) / 4) + 16;
+  // tile, block fields [G][64], flip and sign words [G], spin bytes [G][64],
+  // init table of per-column spin bytes
+  return (size_t)64 * 64 * jbytes + (size_t)G * 64 * fbytes + 2 * 8 * (size_t)G +
+         (size_t)G * 64 + 4 * (size_t)((n + 3) / 4) + 16;
 }
 
 template <typename FT, typename JT, int NS, int G>
