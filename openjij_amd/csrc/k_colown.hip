@@ -326,10 +326,11 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
             const FT dE = s2 * fk;
             const uint64_t cand = ballot64((TT)dE < th) & live;
             if (cand == 0) break;
+            const uint64_t low = cand & (~cand + 1ull);  // the next flip, i0
+            live &= ~(cand ^ (cand - 1ull));              // lanes <= i0 are done
+            flips |= low;
             const int i0 = __builtin_ctzll(cand);
-            live &= lanes_above(i0);
-            const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
-            flips |= 1ull << i0;
+            const FT c = coef2<FT>((sb & low) != 0ull);
             fk = fma_ft<FT>(c, Dc.dyn(i0), fk);
           }
         }

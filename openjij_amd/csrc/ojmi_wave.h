@@ -29,19 +29,21 @@ struct FieldVec {
 
   __device__ __forceinline__ FT get(int m) const { return v[m / VL][m % VL]; }
   __device__ __forceinline__ void set(int m, FT x) { v[m / VL][m % VL] = x; }
-  // k is wave-uniform.  readfirstlane makes that explicit to the compiler so
-  // the access becomes s_set_gpr_idx indirect addressing on a scalar index
-  // (without it, index arithmetic placed in VGPRs turns into a waterfall loop).
+  // k is wave-uniform and in [0, NS).  readfirstlane makes that explicit to
+  // the compiler so the access becomes s_set_gpr_idx indirect addressing on a
+  // scalar index (without it, index arithmetic placed in VGPRs turns into a
+  // waterfall loop); unsigned arithmetic keeps the vector / element split to a
+  // shift and a mask.
   __device__ __forceinline__ FT dyn(int k) const {
-    const int ku = __builtin_amdgcn_readfirstlane(k);
+    const unsigned ku = (unsigned)__builtin_amdgcn_readfirstlane(k);
     if constexpr (NV == 1) {
       return v[0][ku];
     } else {
-      const int q = ku / VL, e = ku - q * VL;
+      const unsigned q = ku / (unsigned)VL, e = ku % (unsigned)VL;
       FT r = v[0][e];
 #pragma unroll
       for (int qq = 1; qq < NV; ++qq)
-        if (q == qq) r = v[qq][e];
+        if (q == (unsigned)qq) r = v[qq][e];
       return r;
     }
   }
