@@ -32,13 +32,23 @@ struct FieldVec {
   // k is wave-uniform and in [0, NS).  readfirstlane makes that explicit to
   // the compiler so the access becomes s_set_gpr_idx indirect addressing on a
   // scalar index (without it, index arithmetic placed in VGPRs turns into a
-  // waterfall loop); unsigned arithmetic keeps the vector / element split to a
-  // shift and a mask.
+  // waterfall loop).  For 32-bit fields an unsigned split keeps the vector /
+  // element arithmetic to a shift and a mask; for FP64 fields that form makes
+  // the compiler move the whole vector to scratch memory, so FP64 keeps the
+  // signed split.
   __device__ __forceinline__ FT dyn(int k) const {
-    const unsigned ku = (unsigned)__builtin_amdgcn_readfirstlane(k);
     if constexpr (NV == 1) {
-      return v[0][ku];
+      return v[0][__builtin_amdgcn_readfirstlane(k)];
+    } else if constexpr (sizeof(FT) == 8) {
+      const int ku = __builtin_amdgcn_readfirstlane(k);
+      const int q = ku / VL, e = ku - q * VL;
+      FT r = v[0][e];
+#pragma unroll
+      for (int qq = 1; qq < NV; ++qq)
+        if (q == qq) r = v[qq][e];
+      return r;
     } else {
+      const unsigned ku = (unsigned)__builtin_amdgcn_readfirstlane(k);
       const unsigned q = ku / (unsigned)VL, e = ku % (unsigned)VL;
       FT r = v[0][e];
 #pragma unroll
@@ -98,14 +108,29 @@ __device__ __forceinline__ int32_t fma_ft<int32_t>(int32_t a, int32_t b, int32_t
   return a * b + c;
 }
 
-// F += c * row  (c = +-2, so c * J is exact and each field sees one rounding)
+// F += c * row  (c = +-2, so c * J is exact and each field sees one rounding).
+// FP64 fields of 64 slots fill 128 VGPRs; a whole FP64 row in flight would take
+// the other 128, so such rows are applied in two halves of 32 slots.
 template <typename JT, typename FT, int NS>
 __device__ __forceinline__ void row_axpy(FieldVec<FT, NS>& F, const char* __restrict__ row,
                                          int lane, FT c) {
-  RowLoad<JT, FT, NS> r;
-  r.load(row, lane);
+  constexpr bool halves = sizeof(FT) == 8 && sizeof(JT) == 8 && NS > 32;
+  if constexpr (halves) {
+    const JT* p = reinterpret_cast<const JT*>(row);
 #pragma unroll
-  for (int m = 0; m < NS; ++m) F.set(m, fma_ft<FT>(c, r.x[m], F.get(m)));
+    for (int m0 = 0; m0 < NS; m0 += 32) {
+      FT x[32];
+#pragma unroll
+      for (int m = 0; m < 32; ++m) x[m] = (FT)p[(m0 + m) * 64 + lane];
+#pragma unroll
+      for (int m = 0; m < 32; ++m) F.set(m0 + m, fma_ft<FT>(c, x[m], F.get(m0 + m)));
+    }
+  } else {
+    RowLoad<JT, FT, NS> r;
+    r.load(row, lane);
+#pragma unroll
+    for (int m = 0; m < NS; ++m) F.set(m, fma_ft<FT>(c, r.x[m], F.get(m)));
+  }
 }
 
 // F = h + sum_k s_k J[k][.], k = 0..n-1 in order (one rounding per term).
