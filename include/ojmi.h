@@ -56,6 +56,7 @@ extern "C" {
 #define OJMI_KERNEL_DENSE_SHARED 4 /* replicas of a workgroup share J rows     */
 #define OJMI_KERNEL_DENSE_COLOWN 5 /* as 4, each wave owns a slice of the fields */
 #define OJMI_KERNEL_BATCH_WAVE 6   /* batch of problems, one wave per replica   */
+#define OJMI_KERNEL_DENSE_LDS 7    /* N > 4096: one workgroup per replica, fields in LDS */
 
 typedef struct ojmi_ctx ojmi_ctx;
 

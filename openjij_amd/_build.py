@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libojmi.so")
 
-HIP_SOURCES = ["ojmi_api.hip", "k_dense.hip", "k_shared.hip", "k_colown.hip", "k_csr.hip", "k_energy.hip"]
+HIP_SOURCES = ["ojmi_api.hip", "k_dense.hip", "k_shared.hip", "k_colown.hip", "k_lds.hip", "k_csr.hip", "k_energy.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

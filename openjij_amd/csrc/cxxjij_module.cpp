@@ -962,6 +962,8 @@ PYBIND11_MODULE(cxxjij, m) {
   me.attr("KERNEL_CSR_LEVEL") = OJMI_KERNEL_CSR_LEVEL;
   me.attr("KERNEL_DENSE_SHARED") = OJMI_KERNEL_DENSE_SHARED;
   me.attr("KERNEL_DENSE_COLOWN") = OJMI_KERNEL_DENSE_COLOWN;
+  me.attr("KERNEL_BATCH_WAVE") = OJMI_KERNEL_BATCH_WAVE;
+  me.attr("KERNEL_DENSE_LDS") = OJMI_KERNEL_DENSE_LDS;
   py::class_<Engine>(me, "Engine", py::module_local())
       .def(py::init([](const DenseGraph& g, int64_t R, int64_t M, double gamma, int prec) {
              return make_engine_dense(g, R, M, gamma, prec);

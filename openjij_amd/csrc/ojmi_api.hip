@@ -170,14 +170,19 @@ int resolve_precision(int32_t req, bool integer, double max_abs_field, int* out)
   return fail(OJMI_EINVAL, "unknown precision");
 }
 
-int common_init(ojmi_ctx* c) {
+// LDS a workgroup may use on the current device (at least 64 KiB).
+int device_lds_budget() {
   int dev = 0, lds = 0, optin = 0;
-  OJMI_HIP(hipGetDevice(&dev));
+  if (hipGetDevice(&dev) != hipSuccess) return 65536;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     lds = 0;
   if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess)
     optin = 0;
-  c->lds_budget = std::max(65536, std::max(lds, optin));
+  return std::max(65536, std::max(lds, optin));
+}
+
+int common_init(ojmi_ctx* c) {
+  c->lds_budget = device_lds_budget();
   OJMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   OJMI_HIP(hipEventCreate(&c->e0));
   OJMI_HIP(hipEventCreate(&c->e1));
@@ -273,8 +278,16 @@ int setup_dense_family(ojmi_ctx* c) {
   c->NS = pow2_slots(n);
   c->Npad = 64LL * c->NS;
   if (c->M == 1) {
-    if (c->NS > 64)
-      return fail(OJMI_ENOTSUP, "dense GPU path supports n <= 4096 spins");
+    if (c->NS > 64) {
+      // fields in LDS, rows of 64 * ceil(n / 64)
+      c->NS = (int)((n + 63) / 64);
+      c->Npad = 64LL * c->NS;
+      const int fb = (c->ftype == FT_F64) ? 8 : 4;
+      if (lds_kernel_bytes((int)c->Npad, fb) > (size_t)device_lds_budget())
+        return fail(OJMI_ENOTSUP, "dense GPU path: n too large for LDS-resident fields");
+      c->family = OJMI_KERNEL_DENSE_LDS;
+      return OJMI_OK;
+    }
     c->family = OJMI_KERNEL_DENSE_WAVE;
   } else {
     const int64_t P = (c->M + 1) / 2;
@@ -490,7 +503,7 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
     delete c;
     return rc;
   }
-  if (c->M == 1 && c->ftype == FT_F32 && c->R >= 8) {
+  if (c->M == 1 && c->ftype == FT_F32 && c->R >= 8 && c->family == OJMI_KERNEL_DENSE_WAVE) {
     // column-owned fields with 4 replicas per workgroup where the row count
     // allows it, else rows staged in LDS for 8 replicas
     c->group = colown_supported((int)c->NS, 4) ? 4 : 8;
@@ -1028,6 +1041,19 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     a.p1_f = c->dp1_f;
     if (c->family == OJMI_KERNEL_SQA_WAVE) {
       e = launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
+    } else if (c->family == OJMI_KERNEL_DENSE_LDS) {
+      if (c->jtype == JT_I8) a.J = c->dJnat;
+      if (c->ftype == FT_F32) {
+        if (!c->dF) {
+          float* f;
+          if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
+          c->dF = f;
+        }
+        a.F_out = c->dF;
+      }
+      c->lds_used = (int)lds_kernel_bytes((int)c->Npad, c->ftype == FT_F64 ? 8 : 4);
+      e = launch_dense_lds(c->ftype, c->jtype, a, (int)c->Npad, c->st);
+      c->fields_valid = (c->ftype == FT_F32);
     } else if (c->family == OJMI_KERNEL_DENSE_SHARED) {
       SharedArgs sa{};
       sa.J = c->dJnat ? c->dJnat : c->dJ;

@@ -98,6 +98,10 @@ hipError_t launch_dense_colown(int ftype, int jtype, const SharedArgs& a, int NS
 
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
 hipError_t launch_batch_wave(int ftype, int jtype, const BatchArgs& b, int NS, hipStream_t st);
+// Classical dense systems too large for register-resident fields (k_lds.hip):
+// natural coupling rows of Npad = 64 * ceil(n / 64), fields in LDS.
+size_t lds_kernel_bytes(int Npad, int fbytes);
+hipError_t launch_dense_lds(int ftype, int jtype, const DenseArgs& a, int Npad, hipStream_t st);
 hipError_t launch_batch_energy(int ftype, int jtype, const BatchArgs& b, const int64_t* rb, int K,
                                double* E, hipStream_t st);
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);

@@ -307,3 +307,18 @@ def test_shared_kernel_xcd_pacing_does_not_change_results(window, colown):
     want = orc.replay_dense("fp32", H.symmetric_ext(E), init, 99, 0,
                             orc.expand_classical(beta, mcs))
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("n,prec", [(4097, "int32"), (5000, "fp64"), (4500, "fp32"),
+                                    (6000, "int32")])
+def test_dense_large_n_fields_in_lds_bitexact(n, prec):
+    """N > 4096: one workgroup per replica with its fields in LDS; the same
+    sequential trajectory (oracle replay, bit for bit) and exact energies."""
+    E = H.int_dense(n, 4000 + n) if prec == "int32" else H.sk_gauss(n, 4000 + n, field=True)
+    beta, mcs = sched(3, 0.2, 3.0)
+    init, got, want, info, en = run_dense(E, prec, 3, beta, mcs, seed=31 + n)
+    assert info["kernel"] == capi.KERNEL_DENSE_LDS
+    assert not np.array_equal(got, init)
+    np.testing.assert_array_equal(got, want)
+    ex = orc.energy_exact(E, got)
+    assert np.all(np.abs(en - ex) <= 1e-9 * (1 + np.abs(ex)))
