@@ -31,9 +31,42 @@ struct CsrThresh<float> {
   typedef float type;
 };
 
-template <typename FT>
+// Neighbour lists are padded to DMAX entries (zero couplings on column 0), so
+// every pass loads the same number of values and the lists of the next pass
+// can be in flight while the current one is processed (DMAX <= 16; wider
+// lists are loaded at the start of their own pass).  A zero coupling adds +-0
+// to the field, which changes no decision.
+template <typename FT, int DMAX>
+struct CsrPass {
+  int i;        // spin index of this lane's position
+  bool valid;   // position inside the level
+  FT hv;
+  int col[DMAX];
+  FT val[DMAX];
+};
+
+template <typename FT, int DMAX>
+__device__ __forceinline__ void csr_load_pass(const CsrArgs& a, int lv, int base, int lane,
+                                              CsrPass<FT, DMAX>& P) {
+  const int b1 = a.lvl_ptr[lv + 1];
+  const int pos = base + lane;
+  P.valid = pos < b1;
+  const int pc = P.valid ? pos : b1 - 1;  // levels are never empty
+  const int n = a.n;
+  P.i = a.order[pc];
+  P.hv = reinterpret_cast<const FT*>(a.h)[pc];
+  const FT* ev = reinterpret_cast<const FT*>(a.val);
+#pragma unroll
+  for (int d = 0; d < DMAX; ++d) {
+    P.col[d] = a.ell_col[(int64_t)d * n + pc];
+    P.val[d] = ev[(int64_t)d * n + pc];
+  }
+}
+
+template <typename FT, int DMAX>
 __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   typedef typename CsrThresh<FT>::type TT;
+  constexpr bool PREFETCH = DMAX <= 16;
   extern __shared__ char smem[];
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -48,12 +81,11 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   int8_t* sp = a.spins + (int64_t)r * n;
   for (int i = lane; i < n; i += 64) s_s[i] = sp[i] < 0 ? (int8_t)-1 : (int8_t)1;
 
-  const FT* h = reinterpret_cast<const FT*>(a.h);
-  const FT* ev = reinterpret_cast<const FT*>(a.val);
   const uint32_t rep = (uint32_t)(a.rep_offset + r);
   const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
-  const int D = a.D;
 
+  CsrPass<FT, DMAX> cur;
+  if constexpr (PREFETCH) csr_load_pass<FT, DMAX>(a, 0, 0, lane, cur);
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
@@ -67,28 +99,59 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
       s_w[4 * g + 2] = w2;
       s_w[4 * g + 3] = w3;
     }
-    for (int lv = 0; lv < a.nlev; ++lv) {
-      const int b0 = a.lvl_ptr[lv], b1 = a.lvl_ptr[lv + 1];
-      for (int base = b0; base < b1; base += 64) {
-        const int pos = base + lane;
-        if (pos < b1) {
-          const int i = a.order[pos];
-          const int dg = a.deg[pos];
-          FT f = h[pos];
-          for (int d = 0; d < dg; ++d) {
-            const int j = a.ell_col[(int64_t)d * n + pos];
-            const FT v = ev[(int64_t)d * n + pos];
-            f = (s_s[j] < 0) ? f - v : f + v;
-          }
-          const int8_t si = s_s[i];
-          const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
-          const TT th = log_uniform<TT>(s_w[i]) * nb;
-          if ((TT)dE < th) s_s[i] = (int8_t)-si;
-        }
+    int lv = 0, base = 0;
+    while (true) {
+      // the pass after this one (the first pass of the next sweep after the last)
+      int nlv = lv, nbase = base + 64;
+      if (nbase >= a.lvl_ptr[lv + 1]) {
+        nlv = lv + 1;
+        nbase = a.lvl_ptr[lv + 1];
       }
+      const bool last = nlv >= a.nlev;
+      if (last) {
+        nlv = 0;
+        nbase = 0;
+      }
+      CsrPass<FT, DMAX> nxt;
+      if constexpr (PREFETCH) {
+        csr_load_pass<FT, DMAX>(a, nlv, nbase, lane, nxt);
+      } else {
+        csr_load_pass<FT, DMAX>(a, lv, base, lane, cur);
+      }
+      if (cur.valid) {
+        FT f = cur.hv;
+#pragma unroll
+        for (int d = 0; d < DMAX; ++d) f = (s_s[cur.col[d]] < 0) ? f - cur.val[d] : f + cur.val[d];
+        const int i = cur.i;
+        const int8_t si = s_s[i];
+        const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
+        const TT th = log_uniform<TT>(s_w[i]) * nb;
+        if ((TT)dE < th) s_s[i] = (int8_t)-si;
+      }
+      if constexpr (PREFETCH) cur = nxt;
+      if (last) break;
+      lv = nlv;
+      base = nbase;
     }
   }
   for (int i = lane; i < n; i += 64) sp[i] = s_s[i];
+}
+
+// ELL width the kernels are instantiated for: the smallest of 8, 16, 32, 64
+// that holds the maximum degree D.
+int csr_padded_width(int D) { return D <= 8 ? 8 : (D <= 16 ? 16 : (D <= 32 ? 32 : 64)); }
+
+template <typename FT>
+static hipError_t launch_csr_ft(const CsrArgs& a, dim3 grid, dim3 block, size_t lds,
+                                hipStream_t st) {
+  switch (a.D) {
+    case 8: hipLaunchKernelGGL((k_csr_level<FT, 8>), grid, block, lds, st, a); break;
+    case 16: hipLaunchKernelGGL((k_csr_level<FT, 16>), grid, block, lds, st, a); break;
+    case 32: hipLaunchKernelGGL((k_csr_level<FT, 32>), grid, block, lds, st, a); break;
+    case 64: hipLaunchKernelGGL((k_csr_level<FT, 64>), grid, block, lds, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, hipStream_t st) {
@@ -96,15 +159,10 @@ hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, hipStream_t st
   const int ng = (a.n + 3) >> 2;
   const size_t lds = (size_t)wpb * ((size_t)ng * 20);
   dim3 grid((a.R + wpb - 1) / wpb), block(64 * wpb);
-  if (ftype == FT_F64)
-    hipLaunchKernelGGL(k_csr_level<double>, grid, block, lds, st, a);
-  else if (ftype == FT_F32)
-    hipLaunchKernelGGL(k_csr_level<float>, grid, block, lds, st, a);
-  else if (ftype == FT_I32)
-    hipLaunchKernelGGL(k_csr_level<int32_t>, grid, block, lds, st, a);
-  else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
+  if (ftype == FT_F64) return launch_csr_ft<double>(a, grid, block, lds, st);
+  if (ftype == FT_F32) return launch_csr_ft<float>(a, grid, block, lds, st);
+  if (ftype == FT_I32) return launch_csr_ft<int32_t>(a, grid, block, lds, st);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace ojmi

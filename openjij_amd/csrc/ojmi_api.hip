@@ -590,8 +590,9 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   pick_types(c, maxabsJ);
   if (c->jtype == JT_I8) c->jtype = JT_I32;
 
-  // Dense-ish graphs (or Trotter systems) run on the dense wave kernels.
-  const bool dense_like = (num_trotter > 1) || (D > 64 && 2 * nnz > n * (int64_t)64);
+  // Trotter systems and graphs with a node of degree > 64 (the widest
+  // neighbour list of the CSR kernel) run on the dense kernels.
+  const bool dense_like = (num_trotter > 1) || (D > 64);
   if (dense_like) {
     std::vector<double> Jn((size_t)(n * n), 0.0);
     for (int64_t i = 0; i < n; ++i)
@@ -624,8 +625,10 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   for (int64_t i = 0; i < n; ++i) lvl_ptr[level[i] + 1]++;
   for (int32_t l = 0; l < nlev; ++l) lvl_ptr[l + 1] += lvl_ptr[l];
   if (D == 0) D = 1;
-  std::vector<int32_t> deg((size_t)n), ell((size_t)(D * n), 0);
-  std::vector<double> ellv((size_t)(D * n), 0.0), hpos((size_t)n);
+  // neighbour lists padded to the kernel's fixed width (zero couplings)
+  const int64_t Dp = csr_padded_width((int)D);
+  std::vector<int32_t> deg((size_t)n), ell((size_t)(Dp * n), 0);
+  std::vector<double> ellv((size_t)(Dp * n), 0.0), hpos((size_t)n);
   for (int64_t pos = 0; pos < n; ++pos) {
     const int64_t i = order[pos];
     deg[pos] = (int32_t)adj[i].size();
@@ -637,7 +640,7 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   }
   c->family = OJMI_KERNEL_CSR_LEVEL;
   c->nlev = nlev;
-  c->D = (int)D;
+  c->D = (int)Dp;
   c->NS = pow2_slots(n);
   c->Npad = 64LL * c->NS;
   // per wave: Philox words (16 B per 4 spins) + int8 spins; <= 64 KiB per block

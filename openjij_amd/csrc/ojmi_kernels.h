@@ -105,6 +105,8 @@ hipError_t launch_dense_lds(int ftype, int jtype, const DenseArgs& a, int Npad, 
 hipError_t launch_batch_energy(int ftype, int jtype, const BatchArgs& b, const int64_t* rb, int K,
                                double* E, hipStream_t st);
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+// ELL width of the CSR kernels for maximum degree D (8, 16, 32 or 64).
+int csr_padded_width(int D);
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, hipStream_t st);
 
 // FP64 fields/energies: H[c][j] = sum_i S[c][i] J64[i][j] (c over C configurations),
