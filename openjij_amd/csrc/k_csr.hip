@@ -137,6 +137,93 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   for (int i = lane; i < n; i += 64) sp[i] = s_s[i];
 }
 
+// Workgroup form: W waves anneal one replica, sharing its spins and the
+// sweep's Philox words in LDS.  Within a level wave w takes the passes at
+// level start + 64*(w + W*j); a barrier ends every level (only LDS writes are
+// waited for, so the prefetched neighbour lists of the next pass stay in
+// flight).  Spins of one level are never neighbours, so the order of the
+// passes inside a level does not matter.
+template <typename FT, int DMAX, int W>
+__global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
+  typedef typename CsrThresh<FT>::type TT;
+  static_assert(DMAX <= 16, "the workgroup form prefetches the next pass");
+  extern __shared__ char smem[];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int r = blockIdx.x;
+  if (r >= a.R) return;  // whole workgroup
+  const int n = a.n;
+  const int ng = (n + 3) >> 2;
+  uint32_t* s_w = reinterpret_cast<uint32_t*>(smem);                      // [4*ng]
+  int8_t* s_s = reinterpret_cast<int8_t*>(smem + (size_t)ng * 16);        // [4*ng]
+  int8_t* sp = a.spins + (int64_t)r * n;
+  for (int i = tid; i < n; i += 64 * W) s_s[i] = sp[i] < 0 ? (int8_t)-1 : (int8_t)1;
+
+  const uint32_t rep = (uint32_t)(a.rep_offset + r);
+  const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
+  auto bar = []() {
+    __builtin_amdgcn_s_waitcnt((0 & 15) | (3 << 14) | (7 << 4) | (0 << 8));  // lgkmcnt(0)
+    __asm__ __volatile__("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ __volatile__("" ::: "memory");
+  };
+
+  CsrPass<FT, DMAX> cur;
+  csr_load_pass<FT, DMAX>(a, 0, 64 * w, lane, cur);
+  for (int t = 0; t < a.T; ++t) {
+    const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
+    TT nb;
+    if constexpr (sizeof(TT) == 8) nb = a.p0_d[t];
+    else nb = a.p0_f[t];
+    for (int g = tid; g < ng; g += 64 * W) {
+      uint32_t w0, w1, w2, w3;
+      philox4x32_10((uint32_t)g, tg, rep, 0u, slo, shi, w0, w1, w2, w3);
+      s_w[4 * g + 0] = w0;
+      s_w[4 * g + 1] = w1;
+      s_w[4 * g + 2] = w2;
+      s_w[4 * g + 3] = w3;
+    }
+    bar();  // words of this sweep and spins of the last are visible
+    int lv = 0, base = 64 * w;
+    while (true) {
+      const int b1 = a.lvl_ptr[lv + 1];
+      int nlv = lv, nbase = base + 64 * W;
+      const bool end_of_level = nbase >= b1;
+      bool last = false;
+      if (end_of_level) {
+        nlv = lv + 1;
+        if (nlv >= a.nlev) {
+          last = true;
+          nlv = 0;
+          nbase = 64 * w;
+        } else {
+          nbase = b1 + 64 * w;
+        }
+      }
+      CsrPass<FT, DMAX> nxt;
+      csr_load_pass<FT, DMAX>(a, nlv, nbase, lane, nxt);
+      if (cur.valid) {
+        FT f = cur.hv;
+#pragma unroll
+        for (int d = 0; d < DMAX; ++d) f = (s_s[cur.col[d]] < 0) ? f - cur.val[d] : f + cur.val[d];
+        const int i = cur.i;
+        const int8_t si = s_s[i];
+        const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
+        const TT th = log_uniform<TT>(s_w[i]) * nb;
+        if ((TT)dE < th) s_s[i] = (int8_t)-si;
+      }
+      cur = nxt;
+      if (end_of_level) bar();  // this level's flips are visible to every wave
+      if (last) break;
+      lv = nlv;
+      base = nbase;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 64 * W) sp[i] = s_s[i];
+}
+
 // ELL width the kernels are instantiated for: the smallest of 8, 16, 32, 64
 // that holds the maximum degree D.
 int csr_padded_width(int D) { return D <= 8 ? 8 : (D <= 16 ? 16 : (D <= 32 ? 32 : 64)); }
@@ -154,8 +241,43 @@ static hipError_t launch_csr_ft(const CsrArgs& a, dim3 grid, dim3 block, size_t 
   return hipGetLastError();
 }
 
+template <typename FT, int W>
+static hipError_t launch_csr_wg_t(const CsrArgs& a, hipStream_t st) {
+  const size_t lds = (size_t)((a.n + 3) >> 2) * 20;
+  if (a.D == 8)
+    hipLaunchKernelGGL((k_csr_wg<FT, 8, W>), dim3(a.R), dim3(64 * W), lds, st, a);
+  else if (a.D == 16)
+    hipLaunchKernelGGL((k_csr_wg<FT, 16, W>), dim3(a.R), dim3(64 * W), lds, st, a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <typename FT>
+static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
+  if (W == 2) return launch_csr_wg_t<FT, 2>(a, st);
+  if (W == 4) return launch_csr_wg_t<FT, 4>(a, st);
+  return hipErrorInvalidValue;
+}
+
+// Waves per replica for the workgroup form (1 = the wave-per-replica kernel):
+// more waves where few replicas leave SIMDs with a single wave.
+int csr_waves_per_replica(int64_t R, int D) {
+  if (D > 16) return 1;
+  if (R <= 512) return 4;
+  if (R <= 2048) return 2;
+  return 1;
+}
+
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, hipStream_t st) {
   if (a.R <= 0 || a.T < 0) return hipSuccess;
+  const int W = csr_waves_per_replica(a.R, a.D);
+  if (W > 1) {
+    if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
+    if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);
+    if (ftype == FT_I32) return launch_csr_wg_ft<int32_t>(a, W, st);
+    return hipErrorInvalidValue;
+  }
   const int ng = (a.n + 3) >> 2;
   const size_t lds = (size_t)wpb * ((size_t)ng * 20);
   dim3 grid((a.R + wpb - 1) / wpb), block(64 * wpb);

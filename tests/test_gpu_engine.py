@@ -102,7 +102,7 @@ def test_config_a_sk256_pm1_replay():
 
 
 # --- CSR --------------------------------------------------------------------
-@pytest.mark.parametrize("L,R,sweeps", [(3, 4, 30), (6, 16, 20), (16, 4, 6)])
+@pytest.mark.parametrize("L,R,sweeps", [(3, 4, 30), (6, 16, 20), (16, 4, 6), (4, 600, 8), (3, 2100, 6)])
 def test_csr_ea3d_int_bitexact(L, R, sweeps):
     n, rp, ci, v = H.ea3d_csr(L, 0xB002 + L)
     beta, mcs = sched(sweeps, 0.1, 5.0)
