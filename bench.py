@@ -293,7 +293,13 @@ def calibrate_copy():
 
 
 KERNEL_NAMES = {1: "k_dense_wave", 2: "k_sqa_wave", 3: "k_csr_level", 4: "k_dense_shared",
-                5: "k_dense_colown", 6: "k_batch_wave"}
+                5: "k_dense_colown", 6: "k_batch_wave", 7: "k_dense_lds"}
+
+
+def kernel_name(info):
+    if info["kernel"] == 3 and info.get("waves_per_replica", 1) > 1:
+        return "k_csr_wg"  # several waves per replica
+    return KERNEL_NAMES.get(info["kernel"], str(info["kernel"]))
 
 
 def main():
@@ -349,7 +355,7 @@ def main():
         pw, pk, e_plain, _, pinfo = time_path(W, W["plain"], 1, 0,
                                               1000 + args.warmup + args.steps - 1)
         plain = dict(value=attempts / float(pw.mean()), kernel_ms=float(pk.mean()),
-                     kernel=KERNEL_NAMES.get(pinfo["kernel"], str(pinfo["kernel"])))
+                     kernel=kernel_name(pinfo))
     else:
         e_plain = e_fast
 
@@ -373,7 +379,7 @@ def main():
         if plain is not None:
             ks_p = H.ks_2samp_pvalue(e_fast, e_plain)
 
-    kname = KERNEL_NAMES.get(info["kernel"], str(info["kernel"]))
+    kname = kernel_name(info)
     bJ = info["coupling_bytes"]
     # one coupling row (deg couplings, plus an int32 column index each for CSR)
     # consulted per attempt

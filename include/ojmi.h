@@ -52,7 +52,7 @@ extern "C" {
 /* kernel family that runs a context (reported by ojmi_get_info) */
 #define OJMI_KERNEL_DENSE_WAVE 1   /* one wave64 per replica, fields in VGPRs   */
 #define OJMI_KERNEL_SQA_WAVE 2     /* one wave64 per Trotter slice              */
-#define OJMI_KERNEL_CSR_LEVEL 3    /* one wave64 per replica, level-parallel   */
+#define OJMI_KERNEL_CSR_LEVEL 3    /* 1-8 wave64s per replica, level-parallel  */
 #define OJMI_KERNEL_DENSE_SHARED 4 /* replicas of a workgroup share J rows     */
 #define OJMI_KERNEL_DENSE_COLOWN 5 /* as 4, each wave owns a slice of the fields */
 #define OJMI_KERNEL_BATCH_WAVE 6   /* batch of problems, one wave per replica   */
@@ -77,6 +77,8 @@ typedef struct ojmi_info {
   int32_t lds_bytes;      /* LDS per workgroup of the last launch            */
   int32_t lds_budget;     /* device LDS limit per workgroup                  */
   int64_t num_problems;   /* problems of a batch context (1 otherwise)       */
+  int32_t waves_per_replica; /* CSR: waves sharing one replica's sweep        */
+  int32_t reserved;
 } ojmi_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -146,6 +148,8 @@ int ojmi_run_batch(ojmi_ctx* ctx, const double* beta, const int64_t* one_mc_step
  *   "round_rows":    shared kernel: rows per staging round (0 = auto)
  *   "colown":        1 (default) = run shared contexts with the column-owned
  *                    kernel where the shape allows it, 0 = LDS row staging
+ *   "csr_waves":     CSR contexts of degree <= 16: waves sharing one replica's
+ *                    sweep (1, 2, 4, 8; 0 = chosen by the replica count)
  */
 int ojmi_set_option(ojmi_ctx* ctx, const char* key, int64_t value);
 

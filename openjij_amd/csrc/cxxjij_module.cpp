@@ -775,6 +775,8 @@ struct Engine {
     d["ring_rows"] = i.ring_rows;
     d["lds_bytes"] = i.lds_bytes;
     d["lds_budget"] = i.lds_budget;
+    d["num_problems"] = i.num_problems;
+    d["waves_per_replica"] = i.waves_per_replica;
     return d;
   }
 };

@@ -257,21 +257,23 @@ template <typename FT>
 static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
   if (W == 2) return launch_csr_wg_t<FT, 2>(a, st);
   if (W == 4) return launch_csr_wg_t<FT, 4>(a, st);
+  if (W == 8) return launch_csr_wg_t<FT, 8>(a, st);
   return hipErrorInvalidValue;
 }
 
 // Waves per replica for the workgroup form (1 = the wave-per-replica kernel):
-// more waves where few replicas leave SIMDs with a single wave.
-int csr_waves_per_replica(int64_t R, int D) {
+// several waves split a level's passes, so each SIMD holds more waves and each
+// wave fewer serial passes.  `requested`
+// (1, 2, 4 or 8) overrides the choice; 0 chooses by R.
+int csr_waves_per_replica(int64_t R, int D, int requested) {
   if (D > 16) return 1;
-  if (R <= 512) return 4;
-  if (R <= 2048) return 2;
-  return 1;
+  if (requested > 0) return requested;
+  return R <= 512 ? 4 : 2;  // measured on 3D EA L=16: 2 waves beat 1 up to R = 4096 at least
 }
 
-hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, hipStream_t st) {
+hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hipStream_t st) {
   if (a.R <= 0 || a.T < 0) return hipSuccess;
-  const int W = csr_waves_per_replica(a.R, a.D);
+  const int W = csr_waves_per_replica(a.R, a.D, waves);
   if (W > 1) {
     if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
     if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);

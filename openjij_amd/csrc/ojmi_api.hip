@@ -63,7 +63,7 @@ struct ojmi_ctx {
   int NS = 1;
   int64_t Npad = 64;
   int sqa_ppw = 1, sqa_w = 1;
-  int csr_wpb = 1;
+  int csr_wpb = 1, csr_waves = 0;
   int refresh_every = 0;
   int group = 1;         // 1: one wave per replica; 4/8/16: replicas per workgroup
   int xcd_window = 2;    // shared kernel: XCD pacing window in blocks (0 = off)
@@ -915,6 +915,12 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     c->round_rows_opt = (int)value;
     return OJMI_OK;
   }
+  if (k == "csr_waves") {
+    if (!(value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
+      return fail(OJMI_EINVAL, "csr_waves must be 0, 1, 2, 4 or 8");
+    c->csr_waves = (int)value;
+    return OJMI_OK;
+  }
   if (k == "xcd_window") {
     if (value < 0 || value > 1000000) return fail(OJMI_EINVAL, "xcd_window must be >= 0");
     c->xcd_window = (int)value;
@@ -1025,7 +1031,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     a.rep_offset = 0;
     a.p0_d = c->dp0_d;
     a.p0_f = c->dp0_f;
-    e = launch_csr_level(c->ftype, a, c->csr_wpb, c->st);
+    e = launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   } else {
     DenseArgs a{};
     a.J = c->dJ;
@@ -1220,6 +1226,9 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->lds_bytes = c->lds_used;
   info->lds_budget = c->lds_budget;
   info->num_problems = (c->kind == 2) ? c->K : 1;
+  info->waves_per_replica =
+      (c->family == OJMI_KERNEL_CSR_LEVEL) ? csr_waves_per_replica(c->R, c->D, c->csr_waves) : 1;
+  info->reserved = 0;
   return OJMI_OK;
 }
 

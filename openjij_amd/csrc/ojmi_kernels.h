@@ -107,7 +107,13 @@ hipError_t launch_batch_energy(int ftype, int jtype, const BatchArgs& b, const i
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
 // ELL width of the CSR kernels for maximum degree D (8, 16, 32 or 64).
 int csr_padded_width(int D);
-hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, hipStream_t st);
+// Waves per replica the CSR launcher uses (1 = one wave per replica, several
+// replicas per workgroup); `requested` 0 chooses by R and D.
+int csr_waves_per_replica(int64_t R, int D, int requested);
+// waves_per_block: replicas per workgroup of the wave-per-replica form;
+// waves: 0 (choose), 1, 2, 4 or 8 waves per replica.
+hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, int waves,
+                            hipStream_t st);
 
 // FP64 fields/energies: H[c][j] = sum_i S[c][i] J64[i][j] (c over C configurations),
 // E[c] = sum_j s_j (H_j/2 + h_j); optional drift = max |F32[c][j] - (H + h)|.

@@ -119,6 +119,21 @@ def test_csr_ea3d_int_bitexact(L, R, sweeps):
         assert en[r] == orc.energy_exact_csr(rp, ci, v, n, got[r])
 
 
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_csr_waves_option_bitexact(waves):
+    n, rp, ci, v = H.ea3d_csr(5, 0xB0C5)
+    beta, mcs = sched(12, 0.1, 4.0)
+    init = H.random_pm1(37, n, 9)
+    with capi.Context.csr(n, rp, ci, v, 37, precision=capi.PREC_FP32) as ctx:
+        ctx.set_option("csr_waves", waves)
+        assert ctx.info()["waves_per_replica"] == waves
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 606, 3)
+        got = ctx.get_spins()
+    want = orc.replay_csr("fp32", rp, ci, v, n, init, 606, 3, orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(got, want)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_csr_real_couplings_bitexact(prec):
     E = H.sk_gauss(90, 4242, field=True)
