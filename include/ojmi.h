@@ -150,6 +150,13 @@ int ojmi_run_batch(ojmi_ctx* ctx, const double* beta, const int64_t* one_mc_step
  *                    kernel where the shape allows it, 0 = LDS row staging
  *   "csr_waves":     CSR contexts of degree <= 16: waves sharing one replica's
  *                    sweep (1, 2, 4, 8; 0 = chosen by the replica count)
+ *   "update_method": acceptance rule of classical contexts (dense, CSR,
+ *                    batch): 0 = METROPOLIS (default), 1 = HEAT_BATH, the
+ *                    reference's algorithm::UpdateMethod values
+ *                    (include/openjij/algorithm/algorithm.hpp:63-76,
+ *                    include/openjij/updater/single_spin_flip.hpp:352-376).
+ *                    Heat bath flips iff dE < log(U / (1 - U)) * (-1/beta),
+ *                    the threshold form of 1 / (1 + exp(beta dE)) > U.
  */
 int ojmi_set_option(ojmi_ctx* ctx, const char* key, int64_t value);
 

@@ -167,7 +167,9 @@ __device__ __forceinline__ void load_chunk(FT (&x)[CS], const JT* __restrict__ p
 
 }  // namespace
 
-template <typename FT, typename JT, int NS, int G>
+// RULE: acceptance rule (RULE_METROPOLIS / RULE_HEAT_BATH) fixed at compile
+// time, so the Metropolis decision code has no rule branch.
+template <typename FT, typename JT, int NS, int G, int RULE>
 __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
   typedef typename ColThresh<FT>::type TT;
   constexpr int CS = NS / G;
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
           for (int q = 0; q < 64; ++q) Dc.set(q, (FT)D[q * 64 + lane]);
           __builtin_amdgcn_sched_barrier(0);
           const int i = 64 * k + lane;
-          const TT th = (i < n) ? log_uniform<TT>(pick4(wd, jj)) * nb : neg_inf<TT>();
+          const TT th = (i < n) ? accept_log<TT>(pick4(wd, jj), RULE) * nb : neg_inf<TT>();
           uint64_t live = ballot64(i < n);
           // A lane is never tested again once it has flipped, so within the
           // block every live lane keeps its sign: dE = s2 * F.
@@ -459,10 +461,10 @@ size_t colown_lds_bytes(int n, int NS, int G, int jbytes, int fbytes) {
          (size_t)G * 64 + 4 * (size_t)((n + 3) / 4) + 16;
 }
 
-template <typename FT, typename JT, int NS, int G>
+template <typename FT, typename JT, int NS, int G, int RULE>
 static hipError_t launch_colown_t(const SharedArgs& a, hipStream_t st) {
   const size_t lds = colown_lds_bytes(a.n, NS, G, (int)sizeof(JT), (int)sizeof(FT));
-  auto kern = k_dense_colown<FT, JT, NS, G>;
+  auto kern = k_dense_colown<FT, JT, NS, G, RULE>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -475,22 +477,27 @@ static hipError_t launch_colown_t(const SharedArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <typename FT, typename JT, int G>
+template <typename FT, typename JT, int G, int RULE>
 static hipError_t launch_colown_g(const SharedArgs& a, int NS, hipStream_t st) {
   switch (NS) {
-    case 8: return launch_colown_t<FT, JT, 8, G>(a, st);
-    case 16: return launch_colown_t<FT, JT, 16, G>(a, st);
-    case 32: return launch_colown_t<FT, JT, 32, G>(a, st);
-    case 64: return launch_colown_t<FT, JT, 64, G>(a, st);
+    case 8: return launch_colown_t<FT, JT, 8, G, RULE>(a, st);
+    case 16: return launch_colown_t<FT, JT, 16, G, RULE>(a, st);
+    case 32: return launch_colown_t<FT, JT, 32, G, RULE>(a, st);
+    case 64: return launch_colown_t<FT, JT, 64, G, RULE>(a, st);
     default: return hipErrorInvalidValue;
   }
 }
 
+// Heat-bath kernels are instantiated for G = 4 only (a replica's trajectory
+// does not depend on G; callers pass G = 4 for such runs).
 template <typename FT, typename JT>
 static hipError_t launch_colown_ft(const SharedArgs& a, int NS, int G, hipStream_t st) {
+  if (a.rule == RULE_HEAT_BATH)
+    return G == 4 ? launch_colown_g<FT, JT, 4, RULE_HEAT_BATH>(a, NS, st) : hipErrorInvalidValue;
+  if (a.rule != RULE_METROPOLIS) return hipErrorInvalidValue;
   switch (G) {
-    case 4: return launch_colown_g<FT, JT, 4>(a, NS, st);
-    case 8: return launch_colown_g<FT, JT, 8>(a, NS, st);
+    case 4: return launch_colown_g<FT, JT, 4, RULE_METROPOLIS>(a, NS, st);
+    case 8: return launch_colown_g<FT, JT, 8, RULE_METROPOLIS>(a, NS, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
         const int i = cur.i;
         const int8_t si = s_s[i];
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
-        const TT th = log_uniform<TT>(s_w[i]) * nb;
+        const TT th = accept_log<TT>(s_w[i], a.rule) * nb;
         if ((TT)dE < th) s_s[i] = (int8_t)-si;
       }
       if constexpr (PREFETCH) cur = nxt;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
         const int i = cur.i;
         const int8_t si = s_s[i];
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
-        const TT th = log_uniform<TT>(s_w[i]) * nb;
+        const TT th = accept_log<TT>(s_w[i], a.rule) * nb;
         if ((TT)dE < th) s_s[i] = (int8_t)-si;
       }
       cur = nxt;

@@ -70,7 +70,7 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
                                                const FT* F_in, FT* F_out, uint32_t rep,
                                                uint32_t slice, uint64_t seed, uint64_t t0,
                                                int T, const typename ThreshT<FT>::type* p0,
-                                               int lane) {
+                                               int rule, int lane) {
   typedef typename ThreshT<FT>::type TT;
   constexpr int64_t RB = row_bytes<JT, NS>();
   uint64_t down = 0;
@@ -100,7 +100,7 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
         const int k = 4 * Q + jj;
         if (k >= NS) break;
         const int i = 64 * k + lane;
-        const TT th = (i < n) ? log_uniform<TT>(w[jj]) * nb : neg_inf<TT>();
+        const TT th = (i < n) ? accept_log<TT>(w[jj], rule) * nb : neg_inf<TT>();
         uint64_t live = ballot64(i < n);
         // a lane is never tested again once it has flipped, so every live
         // lane keeps its sign within the slot: dE = s2 * F
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(256) void k_dense_wave(DenseArgs a) {
   anneal_replica<FT, JT, NS>(reinterpret_cast<const char*>(a.J), reinterpret_cast<const FT*>(a.h),
                              a.spins + (int64_t)r * a.n, a.n, F_in, F_out,
                              (uint32_t)(a.rep_offset + r), 0u, a.seed, a.t0, a.T,
-                             p0_ptr<TT>(a.p0_d, a.p0_f), lane);
+                             p0_ptr<TT>(a.p0_d, a.p0_f), a.rule, lane);
 }
 
 // Batch of independent problems of different sizes (the same slot class NS):
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_batch_wave(BatchArgs b) {
   int8_t* sp = b.spins + b.s_off[k] + (int64_t)r * n;
   const TT* p0 = p0_ptr<TT>(b.p0_d, b.p0_f) + (int64_t)k * b.T;
   anneal_replica<FT, JT, NS>(J, h, sp, n, nullptr, nullptr, (uint32_t)r, (uint32_t)k, b.seed,
-                             b.t0, b.T, p0, lane);
+                             b.t0, b.T, p0, b.rule, lane);
 }
 
 // ---------------------------------------------------------------------------

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(LDS_THREADS) void k_dense_lds(DenseArgs a, int Npad
         }
         const int jj = kb & 3;
         const uint32_t wj = jj == 0 ? wd[0] : (jj == 1 ? wd[1] : (jj == 2 ? wd[2] : wd[3]));
-        const TT th = (i < n) ? log_uniform<TT>(wj) * nb : neg_inf<TT>();
+        const TT th = (i < n) ? accept_log<TT>(wj, a.rule) * nb : neg_inf<TT>();
         FieldVec<FT, 64> Dc;  // Dc[q] = J[64kb + q][64kb + lane]
         const JT* tile = J + (int64_t)(64 * kb) * Npad + 64 * kb + lane;
 #pragma unroll

@@ -324,7 +324,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_shared(SharedArgs a) {
             for (int q = 0; q < 64; ++q) Dc.set(q, (FT)D[q * 64 + lane]);
             __builtin_amdgcn_sched_barrier(0);
           }
-          const TT th = (i < n) ? log_uniform<TT>(wd[jj]) * nb : neg_inf<TT>();
+          const TT th = (i < n) ? accept_log<TT>(wd[jj], a.rule) * nb : neg_inf<TT>();
           uint64_t live = ballot64(i < n);
           uint64_t sb = ballot64((down >> k) & 1ull);
           sb0 = sb;

@@ -64,6 +64,7 @@ struct ojmi_ctx {
   int64_t Npad = 64;
   int sqa_ppw = 1, sqa_w = 1;
   int csr_wpb = 1, csr_waves = 0;
+  int rule = RULE_METROPOLIS;  // classical acceptance rule
   int refresh_every = 0;
   int group = 1;         // 1: one wave per replica; 4/8/16: replicas per workgroup
   int xcd_window = 2;    // shared kernel: XCD pacing window in blocks (0 = off)
@@ -410,6 +411,7 @@ static int run_batch(ojmi_ctx* c, const std::vector<double>& p0 /* [K][T] */, in
   b.t0 = sweep_offset;
   b.p0_d = c->db_p0_d;
   b.p0_f = c->db_p0_f;
+  b.rule = c->rule;
   OJMI_HIP(hipEventRecord(c->e0, c->st));
   // largest slot class first: the longest waves start first
   for (size_t q = c->bclass.size(); q-- > 0;) {
@@ -915,6 +917,17 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     c->round_rows_opt = (int)value;
     return OJMI_OK;
   }
+  if (k == "update_method") {
+    // 0 = METROPOLIS, 1 = HEAT_BATH, the first two values of
+    // algorithm::UpdateMethod (include/openjij/algorithm/algorithm.hpp:63-76),
+    // whose heat-bath rule exists only for classical sweeps.
+    if (value != RULE_METROPOLIS && value != RULE_HEAT_BATH)
+      return fail(OJMI_EINVAL, "update_method must be 0 (METROPOLIS) or 1 (HEAT_BATH)");
+    if (c->M != 1 && value != RULE_METROPOLIS)
+      return fail(OJMI_EINVAL, "HEAT_BATH applies to classical systems (num_trotter = 1)");
+    c->rule = (int)value;
+    return OJMI_OK;
+  }
   if (k == "csr_waves") {
     if (!(value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
       return fail(OJMI_EINVAL, "csr_waves must be 0, 1, 2, 4 or 8");
@@ -1031,6 +1044,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     a.rep_offset = 0;
     a.p0_d = c->dp0_d;
     a.p0_f = c->dp0_f;
+    a.rule = c->rule;
     e = launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   } else {
     DenseArgs a{};
@@ -1048,6 +1062,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     a.p0_f = c->dp0_f;
     a.p1_d = c->dp1_d;
     a.p1_f = c->dp1_f;
+    a.rule = c->rule;
     if (c->family == OJMI_KERNEL_SQA_WAVE) {
       e = launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
     } else if (c->family == OJMI_KERNEL_DENSE_LDS) {
@@ -1073,6 +1088,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
       sa.T = (int)T;
       sa.seed = seed;
       sa.t0 = sweep_offset;
+      sa.rule = c->rule;
       sa.rep_offset = 0;
       sa.p0_d = c->dp0_d;
       sa.p0_f = c->dp0_f;
@@ -1084,7 +1100,12 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
         }
         sa.F_out = c->dF;
       }
-      const int G = c->group;
+      // heat-bath colown kernels exist for G = 4 only; a replica's trajectory
+      // does not depend on G, so such runs take G = 4 (grid and pacing below too)
+      const bool hb_colown = c->rule == RULE_HEAT_BATH && c->colown &&
+                             colown_supported((int)c->NS, c->group) &&
+                             colown_supported((int)c->NS, 4);
+      const int G = hb_colown ? 4 : c->group;
       const int jb = (c->jtype == JT_I8) ? 1 : 4;
       const bool use_colown = c->colown && colown_supported((int)c->NS, G);
       const size_t fixed = shared_lds_bytes((int)c->NS, G, 0, jb);

@@ -110,6 +110,21 @@ __device__ __forceinline__ float log_uniform<float>(uint32_t u) {
   return det_log_f(uniform_f(u));
 }
 
+// log of the acceptance threshold's random factor for rule 0 (Metropolis:
+// flip iff dE < log(U) * (-1/beta), single_spin_flip.hpp:80-82) or rule 1
+// (heat bath: flip iff 1 / (1 + exp(beta dE)) > U, single_spin_flip.hpp:365-371,
+// i.e. dE < log(U / (1 - U)) * (-1/beta)).  1 - U is exactly the uniform of
+// ~u in both precisions (U = (u + 1/2) 2^-32, or ((u >> 9) + 1/2) 2^-23), so
+// log(1 - U) is the same deterministic logarithm of another word.
+constexpr int RULE_METROPOLIS = 0;
+constexpr int RULE_HEAT_BATH = 1;
+template <typename TT>
+__device__ __forceinline__ TT accept_log(uint32_t u, int rule) {
+  TT x = log_uniform<TT>(u);
+  if (rule == RULE_HEAT_BATH) x = x - log_uniform<TT>(~u);
+  return x;
+}
+
 // ---------------------------------------------------------------------------
 // wave64 helpers
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }

@@ -25,6 +25,7 @@ struct DenseArgs {
   const float* p1_f;
   const void* F_in;    // optional [R][Npad] initial fields (field type)
   void* F_out;         // optional [R][Npad] final fields
+  int rule;            // classical acceptance rule: RULE_METROPOLIS / RULE_HEAT_BATH
 };
 
 // Batch of independent dense classical problems (k_dense.hip, k_batch_wave):
@@ -43,6 +44,7 @@ struct BatchArgs {
   uint64_t seed, t0;
   const double* p0_d;    // [K][T] -1/beta per sweep
   const float* p0_f;
+  int rule;
 };
 
 // CSR in level order (k_csr.hip): spins sorted by level, neighbours in ELL
@@ -62,6 +64,7 @@ struct CsrArgs {
   int rep_offset;
   const double* p0_d;
   const float* p0_f;
+  int rule;
 };
 
 // Workgroup-shared dense sweep (k_shared.hip): J in natural row layout
@@ -82,6 +85,7 @@ struct SharedArgs {
   int xcd_window;       // 0: no pacing; else max lead (blocks) over XCD peers
   int xcd_spin;         // bound on pacing polls per block
   uint32_t* progress;   // [workgroups] blocks completed
+  int rule;
 };
 
 size_t shared_lds_bytes(int NS, int G, int ring_rows, int fbytes);

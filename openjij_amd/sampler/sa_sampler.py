@@ -229,6 +229,30 @@ class SASampler(BaseSampler):
         resp.info["schedule"] = self.schedule_info
         return resp
 
+    def sample_hubo(self, J, vartype: Optional[str] = None, num_sweeps: int = 1000,
+                    num_reads: int = 1, num_threads: int = 1,
+                    beta_min: Optional[float] = None, beta_max: Optional[float] = None,
+                    updater: str = "METROPOLIS", random_number_engine: str = "XORSHIFT",
+                    seed: Optional[int] = None, temperature_schedule: str = "GEOMETRIC",
+                    precision=None) -> Response:
+        """Polynomial (dict) models whose terms reduce to degree <= 2, with the
+        METROPOLIS or HEAT_BATH rule and a GEOMETRIC or LINEAR schedule
+        (openjij/sampler/sa_sampler.py:487-567).  The reference's "k-local"
+        updater and non-dict inputs take its older path, which is not part of
+        this engine."""
+        from openjij_amd.sampler.hubo import sample_hubo_quadratic
+
+        if updater == "k-local" or not isinstance(J, dict):
+            raise NotImplementedError(
+                "the k-local updater (higher-order models) is not part of this engine")
+        if updater == "single spin flip":
+            updater = "METROPOLIS"
+        return sample_hubo_quadratic(
+            J, vartype=vartype, num_sweeps=num_sweeps, num_reads=num_reads,
+            num_threads=num_threads, beta_min=beta_min, beta_max=beta_max,
+            update_method=updater, random_number_engine=random_number_engine, seed=seed,
+            temperature_schedule=temperature_schedule, precision=precision)
+
     def sample_batch(self, models, beta_min: Optional[float] = None,
                      beta_max: Optional[float] = None, num_sweeps: Optional[int] = None,
                      num_reads: Optional[int] = None, seed: Optional[int] = None,

@@ -38,6 +38,9 @@
 //   ref_sa_* / ref_sqa -> the 5 ground-state fixtures of tests/cxxtest/testcase.hpp,
 //                         brute force, invariant dE = -2 s (J s) after every run
 //   replay_*           -> brute force, invariants, KS vs ref_* (statistical)
+//   theta_hb_* (heat-bath rule) -> the reference's form 1/(1+exp(beta dE)) > U
+//                         evaluated directly, the closed-form acceptance
+//                         probability, Boltzmann stationarity by enumeration
 //   energy_exact       -> closed forms of tests/cxxtest/graph/quadratic.hpp
 //
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off -fopenmp).
@@ -179,6 +182,21 @@ inline double uniform_d(u32 u) { return ((double)u + 0.5) * 0x1p-32; }
 inline float uniform_f(u32 u) { return ((float)(u >> 9) + 0.5f) * 0x1p-23f; }
 inline double theta_d(u32 u, double nbinv) { return det_log_d(uniform_d(u)) * nbinv; }
 inline float theta_f(u32 u, float nbinv) { return det_log_f(uniform_f(u)) * nbinv; }
+// Heat-bath threshold: the reference flips iff 1 / (1 + exp(beta dE)) > U
+// (include/openjij/updater/single_spin_flip.hpp:365-371), i.e. iff
+// dE < log(U / (1 - U)) / beta.  1 - U = uniform(~u) exactly, in both precisions.
+inline double theta_hb_d(u32 u, double nbinv) {
+  return (det_log_d(uniform_d(u)) - det_log_d(uniform_d(~u))) * nbinv;
+}
+inline float theta_hb_f(u32 u, float nbinv) {
+  return (det_log_f(uniform_f(u)) - det_log_f(uniform_f(~u))) * nbinv;
+}
+inline double theta_rule_d(int rule, u32 u, double nbinv) {
+  return rule ? theta_hb_d(u, nbinv) : theta_d(u, nbinv);
+}
+inline float theta_rule_f(int rule, u32 u, float nbinv) {
+  return rule ? theta_hb_f(u, nbinv) : theta_f(u, nbinv);
+}
 
 // double-double accumulation (Knuth TwoSum); terms here are exact products.
 struct DD {
@@ -211,6 +229,8 @@ double orc_det_log_d(double x) { return det_log_d(x); }
 float orc_det_log_f(float x) { return det_log_f(x); }
 double orc_theta_d(u32 u, double nbinv) { return theta_d(u, nbinv); }
 float orc_theta_f(u32 u, float nbinv) { return theta_f(u, nbinv); }
+double orc_theta_hb_d(u32 u, double nbinv) { return theta_hb_d(u, nbinv); }
+float orc_theta_hb_f(u32 u, float nbinv) { return theta_hb_f(u, nbinv); }
 
 // Reference Xorshift raw stream (for pinning against the recurrence).
 void orc_ref_xorshift(unsigned seed, i64 count, u32* out) {
@@ -542,13 +562,13 @@ int orc_brute_force_dense(i64 n, const double* J, double* emin, i8* best) {
 // Fields F_j = h_j + sum_k J_jk s_k are initialised lane-sequentially:
 // F = h; for k = 0..n-1: F += s_k * J[k][.]; a flip of spin i adds
 // -2 s_i(old) * J[i][.].  The spin of index i is accepted iff
-// dE_i = -2 s_i F_i < theta(u(r,t,0,i)).
+// dE_i = -2 s_i F_i < theta(u(r,t,0,i)); rule 1 (heat bath) uses theta_hb.
 // nbinv[t] = -1/beta_t for each expanded sweep t (fp32 modes use (float)nbinv).
 
 }  // extern "C"
 template <typename F>
 static void replay_dense_T(int mode, i64 n, const double* J, i64 R, i8* spins, u64 seed,
-                           u64 t0, const double* nbinv, i64 T, u32 word3) {
+                           u64 t0, const double* nbinv, i64 T, u32 word3, int rule) {
   const i64 n1 = n + 1;
   std::vector<F> Jc((size_t)n * n), h(n);
   for (i64 i = 0; i < n; ++i) {
@@ -576,8 +596,8 @@ static void replay_dense_T(int mode, i64 n, const double* J, i64 R, i8* spins, u
         const u32 u = philox_word(seed, tg, r, word3, i);
         const F dE = (F)(-2 * s[i]) * Fv[i];
         bool acc;
-        if (mode == 1) acc = (float)dE < theta_f(u, nbf);
-        else acc = (double)dE < theta_d(u, nb);
+        if (mode == 1) acc = (float)dE < theta_rule_f(rule, u, nbf);
+        else acc = (double)dE < theta_rule_d(rule, u, nb);
         if (acc) {
           const F c = (F)(-2 * s[i]);
           const F* row = &Jc[i * n];
@@ -592,18 +612,24 @@ extern "C" {
 
 // word3: the fourth Philox counter word (0 for a single problem; the problem
 // index inside a batch of problems).
-int orc_replay_dense_w(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64 seed,
-                       u64 sweep_offset, const double* nbinv, i64 T, u32 word3) {
-  if (n < 1 || R < 0 || T < 0) return 1;
+// rule: 0 = Metropolis, 1 = heat bath.
+int orc_replay_dense_wr(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64 seed,
+                        u64 sweep_offset, const double* nbinv, i64 T, u32 word3, int rule) {
+  if (n < 1 || R < 0 || T < 0 || rule < 0 || rule > 1) return 1;
   if (mode == 0)
-    replay_dense_T<double>(0, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3);
+    replay_dense_T<double>(0, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, rule);
   else if (mode == 1)
-    replay_dense_T<float>(1, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3);
+    replay_dense_T<float>(1, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, rule);
   else if (mode == 2) {
     if (!is_integer_matrix(J_ext, (n + 1) * (n + 1))) return 2;
-    replay_dense_T<int32_t>(2, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3);
+    replay_dense_T<int32_t>(2, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, rule);
   } else return 1;
   return 0;
+}
+
+int orc_replay_dense_w(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64 seed,
+                       u64 sweep_offset, const double* nbinv, i64 T, u32 word3) {
+  return orc_replay_dense_wr(mode, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, 0);
 }
 
 int orc_replay_dense(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64 seed,
@@ -618,7 +644,7 @@ int orc_replay_dense(int mode, i64 n, const double* J_ext, i64 R, i8* spins, u64
 }  // extern "C"
 template <typename F>
 static void replay_csr_T(int mode, i64 n, const i64* rp, const i64* ci, const double* v, i64 R,
-                         i8* spins, u64 seed, u64 t0, const double* nbinv, i64 T) {
+                         i8* spins, u64 seed, u64 t0, const double* nbinv, i64 T, int rule) {
   std::vector<F> h(n, (F)0);
   std::vector<i64> rp2(n + 1, 0), c2;
   std::vector<F> v2;
@@ -649,8 +675,8 @@ static void replay_csr_T(int mode, i64 n, const i64* rp, const i64* ci, const do
         const u32 u = philox_word(seed, tg, r, 0, i);
         const F dE = (F)(-2 * s[i]) * acc;
         bool accp;
-        if (mode == 1) accp = (float)dE < theta_f(u, nbf);
-        else accp = (double)dE < theta_d(u, nb);
+        if (mode == 1) accp = (float)dE < theta_rule_f(rule, u, nbf);
+        else accp = (double)dE < theta_rule_d(rule, u, nb);
         if (accp) s[i] = (i8)-s[i];
       }
     }
@@ -658,16 +684,24 @@ static void replay_csr_T(int mode, i64 n, const i64* rp, const i64* ci, const do
 }
 extern "C" {
 
-int orc_replay_csr(int mode, i64 n, const i64* rowptr, const i64* col, const double* val, i64 R,
-                   i8* spins, u64 seed, u64 sweep_offset, const double* nbinv, i64 T) {
-  if (n < 1 || R < 0 || T < 0) return 1;
-  if (mode == 0) replay_csr_T<double>(0, n, rowptr, col, val, R, spins, seed, sweep_offset, nbinv, T);
-  else if (mode == 1) replay_csr_T<float>(1, n, rowptr, col, val, R, spins, seed, sweep_offset, nbinv, T);
+int orc_replay_csr_r(int mode, i64 n, const i64* rowptr, const i64* col, const double* val,
+                     i64 R, i8* spins, u64 seed, u64 sweep_offset, const double* nbinv, i64 T,
+                     int rule) {
+  if (n < 1 || R < 0 || T < 0 || rule < 0 || rule > 1) return 1;
+  if (mode == 0)
+    replay_csr_T<double>(0, n, rowptr, col, val, R, spins, seed, sweep_offset, nbinv, T, rule);
+  else if (mode == 1)
+    replay_csr_T<float>(1, n, rowptr, col, val, R, spins, seed, sweep_offset, nbinv, T, rule);
   else if (mode == 2) {
     if (!is_integer_matrix(val, rowptr[n + 1])) return 2;
-    replay_csr_T<int32_t>(2, n, rowptr, col, val, R, spins, seed, sweep_offset, nbinv, T);
+    replay_csr_T<int32_t>(2, n, rowptr, col, val, R, spins, seed, sweep_offset, nbinv, T, rule);
   } else return 1;
   return 0;
+}
+
+int orc_replay_csr(int mode, i64 n, const i64* rowptr, const i64* col, const double* val, i64 R,
+                   i8* spins, u64 seed, u64 sweep_offset, const double* nbinv, i64 T) {
+  return orc_replay_csr_r(mode, n, rowptr, col, val, R, spins, seed, sweep_offset, nbinv, T, 0);
 }
 
 // Trotter replay, dense.  spins: R x M x n.  Per sweep t: a[t] = s*(beta/M),

@@ -32,6 +32,8 @@ def lib():
             "orc_det_log_f": (flt, [flt]),
             "orc_theta_d": (dbl, [u32, dbl]),
             "orc_theta_f": (flt, [u32, flt]),
+            "orc_theta_hb_d": (dbl, [u32, dbl]),
+            "orc_theta_hb_f": (flt, [u32, flt]),
             "orc_ref_xorshift": (None, [C.c_uint, i64, vp]),
             "orc_ref_uniform": (None, [C.c_uint, i64, vp]),
             "orc_make_classical_schedule": (None, [dbl, dbl, i64, vp]),
@@ -50,7 +52,11 @@ def lib():
             "orc_brute_force_dense": (C.c_int, [i64, vp, vp, vp]),
             "orc_replay_dense": (C.c_int, [C.c_int, i64, vp, i64, vp, u64, u64, vp, i64]),
             "orc_replay_dense_w": (C.c_int, [C.c_int, i64, vp, i64, vp, u64, u64, vp, i64, u32]),
+            "orc_replay_dense_wr": (C.c_int, [C.c_int, i64, vp, i64, vp, u64, u64, vp, i64, u32,
+                                              C.c_int]),
             "orc_replay_csr": (C.c_int, [C.c_int, i64, vp, vp, vp, i64, vp, u64, u64, vp, i64]),
+            "orc_replay_csr_r": (C.c_int, [C.c_int, i64, vp, vp, vp, i64, vp, u64, u64, vp, i64,
+                                           C.c_int]),
             "orc_replay_sqa_dense": (C.c_int, [C.c_int, i64, vp, i64, i64, vp, u64, u64, vp, vp, i64]),
             "orc_replay_sqa_csr": (C.c_int, [C.c_int, i64, vp, vp, vp, i64, i64, vp, u64, u64, vp, vp, i64]),
         }
@@ -208,23 +214,24 @@ def brute_force(J_ext):
     return e.value, st
 
 
-def replay_dense(mode, J_ext, spins, seed, sweep_offset, nbinv, word3=0):
-    """word3: fourth Philox counter word (the problem index inside a batch)."""
+def replay_dense(mode, J_ext, spins, seed, sweep_offset, nbinv, word3=0, rule=0):
+    """word3: fourth Philox counter word (the problem index inside a batch);
+    rule: 0 = Metropolis, 1 = heat bath."""
     J = _f64(J_ext)
     s = _i8(spins).copy()
     nb = _f64(nbinv)
-    rc = lib().orc_replay_dense_w(MODE[mode], J.shape[0] - 1, _p(J), s.shape[0], _p(s), seed,
-                                  sweep_offset, _p(nb), len(nb), word3)
+    rc = lib().orc_replay_dense_wr(MODE[mode], J.shape[0] - 1, _p(J), s.shape[0], _p(s), seed,
+                                   sweep_offset, _p(nb), len(nb), word3, rule)
     assert rc == 0, rc
     return s
 
 
-def replay_csr(mode, rowptr, col, val, n, spins, seed, sweep_offset, nbinv):
+def replay_csr(mode, rowptr, col, val, n, spins, seed, sweep_offset, nbinv, rule=0):
     rp, ci, v = _i64(rowptr), _i64(col), _f64(val)
     s = _i8(spins).copy()
     nb = _f64(nbinv)
-    rc = lib().orc_replay_csr(MODE[mode], n, _p(rp), _p(ci), _p(v), s.shape[0], _p(s), seed,
-                              sweep_offset, _p(nb), len(nb))
+    rc = lib().orc_replay_csr_r(MODE[mode], n, _p(rp), _p(ci), _p(v), s.shape[0], _p(s), seed,
+                                sweep_offset, _p(nb), len(nb), rule)
     assert rc == 0, rc
     return s
 

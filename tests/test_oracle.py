@@ -312,3 +312,85 @@ def test_replay_fourth_counter_word_selects_the_stream():
                 s = -s
         got = orc.replay_dense("fp64", J, np.array([[1]], np.int8), seed, 0, nb, word3=word3)
         assert got[0, 0] == s
+
+
+# --- heat-bath rule -----------------------------------------------------------
+def _hb_words(count, seed):
+    return (H.splitmix64_np(seed, count) >> np.uint64(32)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_heat_bath_threshold_is_the_reference_condition(prec):
+    """dE < theta_hb(u, -1/beta) must decide exactly as the reference's
+    1 / (1 + exp(beta dE)) > U (single_spin_flip.hpp:365-371), evaluated here
+    directly in extended precision, except within rounding of a tie."""
+    lib = orc.lib()
+    f = lib.orc_theta_hb_d if prec == "fp64" else lib.orc_theta_hb_f
+    words = _hb_words(60000, 0x4EA7)
+    rng = np.random.default_rng(5)
+    betas = rng.choice([0.05, 0.3, 1.0, 2.5, 7.0], words.size)
+    bde = rng.uniform(-14.0, 14.0, words.size)  # beta * dE
+    bde[:200] = 0.0                              # dE = 0: accepted with probability 1/2
+    dE = bde / betas
+    if prec == "fp64":
+        U = (words.astype(np.longdouble) + 0.5) * np.longdouble(2.0) ** -32
+        tie = 1e-9
+    else:
+        U = ((words >> np.uint32(9)).astype(np.longdouble) + 0.5) * np.longdouble(2.0) ** -23
+        tie = 1e-4
+        dE = dE.astype(np.float32).astype(np.float64)
+        bde = dE * betas
+    ref = 1.0 / (1.0 + np.exp(np.longdouble(1.0) * betas * dE)) > U
+    got = np.array([dE[k] < f(int(words[k]), -1.0 / betas[k]) for k in range(words.size)])
+    logit = np.log(U / (1.0 - U))
+    far = np.abs(logit - bde.astype(np.longdouble)) > tie * (1.0 + np.abs(logit))
+    assert far.sum() > 0.99 * words.size
+    np.testing.assert_array_equal(got[far], ref[far])
+    # a zero-energy move is accepted about half of the time (Metropolis: always)
+    assert 0.4 < got[:200].mean() < 0.6
+
+
+@pytest.mark.parametrize("mode", ["fp64", "fp32", "int32"])
+def test_heat_bath_flip_probability_closed_form(mode):
+    """One spin with field h, one sweep: P(flip) = 1 / (1 + exp(beta dE)),
+    dE = -2 s h, over 40000 independent replicas (5 sigma)."""
+    R = 40000
+    for h, beta in ((0.3, 1.7), (-0.5, 1.7), (2.0, 0.25), (0.0, 3.0)):
+        if mode == "int32":
+            h = float(round(h * 2))
+        J = np.array([[0.0, h], [h, 1.0]])
+        init = np.ones((R, 1), np.int8)
+        out = orc.replay_dense(mode, J, init, 0xC0FFEE, 0, np.array([-1.0 / beta]), rule=1)
+        p = 1.0 / (1.0 + math.exp(beta * (-2.0 * h)))
+        freq = float((out[:, 0] < 0).mean())
+        assert abs(freq - p) < 5 * math.sqrt(p * (1 - p) / R) + 1e-12, (h, beta, freq, p)
+        met = orc.replay_dense(mode, J, init, 0xC0FFEE, 0, np.array([-1.0 / beta]), rule=0)
+        pm = min(1.0, math.exp(-beta * (-2.0 * h)))
+        assert abs(float((met[:, 0] < 0).mean()) - pm) < 5 * math.sqrt(pm * (1 - pm) / R) + 1e-12
+
+
+def test_heat_bath_sweeps_sample_the_boltzmann_distribution():
+    """Sequential heat-bath sweeps leave the Boltzmann distribution invariant:
+    after 30 sweeps at fixed beta, 40000 replicas of a 3-spin system match the
+    enumerated exp(-beta E) / Z (chi-square, p > 1e-4), dense and CSR."""
+    from scipy.stats import chi2
+
+    n, beta, R = 3, 1.1, 40000
+    Ju = np.zeros((n + 1, n + 1))
+    Ju[0, 1], Ju[0, 2], Ju[1, 2] = 0.4, -0.7, 0.25
+    Ju[0, n], Ju[1, n], Ju[2, n] = 0.3, -0.2, 0.1
+    Ju[n, n] = 1.0
+    E = H.symmetric_ext(Ju)
+    states = np.array([[1 - 2 * ((k >> i) & 1) for i in range(n)] for k in range(2 ** n)], np.int8)
+    en = np.array([orc.energy_exact(E, s) for s in states])
+    w = np.exp(-beta * (en - en.min()))
+    p = w / w.sum()
+    nb = np.full(30, -1.0 / beta)
+    init = H.random_pm1(R, n, 77)
+    rp, ci, v = orc.dense_to_csr(E)
+    for out in (orc.replay_dense("fp64", E, init, 12345, 0, nb, rule=1),
+                orc.replay_csr("fp64", rp, ci, v, n, init, 12345, 0, nb, rule=1)):
+        idx = ((out < 0).astype(np.int64) * (1 << np.arange(n))).sum(axis=1)
+        cnt = np.bincount(idx, minlength=2 ** n)
+        stat = float((((cnt - R * p) ** 2) / (R * p)).sum())
+        assert chi2.sf(stat, 2 ** n - 1) > 1e-4, (cnt, R * p)
