@@ -16,7 +16,8 @@
 // Fields are recomputed per attempt in stored CSR order,
 //   F_i = h_i + sum_e val_e s_col(e),
 // from spins kept as int8 in LDS; the Philox words of one sweep are generated
-// four per lane up front into LDS (counter (i>>2, t, r, 0), word i&3).
+// four per lane up front into LDS (counter (i>>2, t, r, 0), word i&3), or, in
+// the workgroup form, the thresholds computed from them.
 #include "ojmi_kernels.h"
 #include "ojmi_device.h"
 
@@ -138,7 +139,8 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
 }
 
 // Workgroup form: W waves anneal one replica, sharing its spins and the
-// sweep's Philox words in LDS.  Within a level wave w takes the passes at
+// sweep's acceptance thresholds in LDS (computed from the Philox words before
+// the first level, so the per-level chain holds no logarithm).  Within a level wave w takes the passes at
 // level start + 64*(w + W*j); a barrier ends every level (only LDS writes are
 // waited for, so the prefetched neighbour lists of the next pass stay in
 // flight).  Spins of one level are never neighbours, so the order of the
@@ -155,8 +157,11 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   if (r >= a.R) return;  // whole workgroup
   const int n = a.n;
   const int ng = (n + 3) >> 2;
-  uint32_t* s_w = reinterpret_cast<uint32_t*>(smem);                      // [4*ng]
-  int8_t* s_s = reinterpret_cast<int8_t*>(smem + (size_t)ng * 16);        // [4*ng]
+  // thresholds of the sweep's attempts, theta_i = log-factor(u_i) * p0[t],
+  // computed for all spins before the levels so that no level waits on a
+  // logarithm; then the spins
+  TT* s_t = reinterpret_cast<TT*>(smem);                                          // [4*ng]
+  int8_t* s_s = reinterpret_cast<int8_t*>(smem + (size_t)ng * 4 * sizeof(TT));    // [4*ng]
   int8_t* sp = a.spins + (int64_t)r * n;
   for (int i = tid; i < n; i += 64 * W) s_s[i] = sp[i] < 0 ? (int8_t)-1 : (int8_t)1;
 
@@ -179,12 +184,12 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
     for (int g = tid; g < ng; g += 64 * W) {
       uint32_t w0, w1, w2, w3;
       philox4x32_10((uint32_t)g, tg, rep, 0u, slo, shi, w0, w1, w2, w3);
-      s_w[4 * g + 0] = w0;
-      s_w[4 * g + 1] = w1;
-      s_w[4 * g + 2] = w2;
-      s_w[4 * g + 3] = w3;
+      s_t[4 * g + 0] = accept_log<TT>(w0, a.rule) * nb;
+      s_t[4 * g + 1] = accept_log<TT>(w1, a.rule) * nb;
+      s_t[4 * g + 2] = accept_log<TT>(w2, a.rule) * nb;
+      s_t[4 * g + 3] = accept_log<TT>(w3, a.rule) * nb;
     }
-    bar();  // words of this sweep and spins of the last are visible
+    bar();  // thresholds of this sweep and spins of the last are visible
     int lv = 0, base = 64 * w;
     while (true) {
       const int b1 = a.lvl_ptr[lv + 1];
@@ -210,8 +215,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
         const int i = cur.i;
         const int8_t si = s_s[i];
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
-        const TT th = accept_log<TT>(s_w[i], a.rule) * nb;
-        if ((TT)dE < th) s_s[i] = (int8_t)-si;
+        if ((TT)dE < s_t[i]) s_s[i] = (int8_t)-si;
       }
       cur = nxt;
       if (end_of_level) bar();  // this level's flips are visible to every wave
@@ -241,16 +245,29 @@ static hipError_t launch_csr_ft(const CsrArgs& a, dim3 grid, dim3 block, size_t 
   return hipGetLastError();
 }
 
+// LDS of the workgroup form: thresholds [4*ng] of `tbytes` each, then the
+// int8 spins [4*ng] (tbytes = sizeof(CsrThresh<FT>::type): 4 for FP32 fields,
+// 8 for FP64 and INT32 fields).
+static size_t csr_wg_lds_bytes(int n, size_t tbytes) {
+  return (size_t)((n + 3) >> 2) * (4 * tbytes + 4);
+}
+
+template <typename FT, int DMAX, int W>
+static hipError_t launch_csr_wg_d(const CsrArgs& a, hipStream_t st) {
+  const size_t lds = csr_wg_lds_bytes(a.n, sizeof(typename CsrThresh<FT>::type));
+  auto kern = k_csr_wg<FT, DMAX, W>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3(a.R), dim3(64 * W), lds, st, a);
+  return hipGetLastError();
+}
+
 template <typename FT, int W>
 static hipError_t launch_csr_wg_t(const CsrArgs& a, hipStream_t st) {
-  const size_t lds = (size_t)((a.n + 3) >> 2) * 20;
-  if (a.D == 8)
-    hipLaunchKernelGGL((k_csr_wg<FT, 8, W>), dim3(a.R), dim3(64 * W), lds, st, a);
-  else if (a.D == 16)
-    hipLaunchKernelGGL((k_csr_wg<FT, 16, W>), dim3(a.R), dim3(64 * W), lds, st, a);
-  else
-    return hipErrorInvalidValue;
-  return hipGetLastError();
+  if (a.D == 8) return launch_csr_wg_d<FT, 8, W>(a, st);
+  if (a.D == 16) return launch_csr_wg_d<FT, 16, W>(a, st);
+  return hipErrorInvalidValue;
 }
 
 template <typename FT>
