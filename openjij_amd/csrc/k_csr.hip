@@ -46,10 +46,12 @@ struct CsrPass {
   FT val[DMAX];
 };
 
+// lp: the level pointers (global memory, or a copy in LDS); lv is
+// wave-uniform, and so is the level end read from it.
 template <typename FT, int DMAX>
-__device__ __forceinline__ void csr_load_pass(const CsrArgs& a, int lv, int base, int lane,
-                                              CsrPass<FT, DMAX>& P) {
-  const int b1 = a.lvl_ptr[lv + 1];
+__device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const int32_t* lp, int lv,
+                                              int base, int lane, CsrPass<FT, DMAX>& P) {
+  const int b1 = __builtin_amdgcn_readfirstlane(lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
   const int pos = base + lane;
   P.valid = pos < b1;
   const int pc = P.valid ? pos : b1 - 1;  // levels are never empty
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
 
   CsrPass<FT, DMAX> cur;
-  if constexpr (PREFETCH) csr_load_pass<FT, DMAX>(a, 0, 0, lane, cur);
+  if constexpr (PREFETCH) csr_load_pass<FT, DMAX>(a, a.lvl_ptr, 0, 0, lane, cur);
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
@@ -115,9 +117,9 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
       }
       CsrPass<FT, DMAX> nxt;
       if constexpr (PREFETCH) {
-        csr_load_pass<FT, DMAX>(a, nlv, nbase, lane, nxt);
+        csr_load_pass<FT, DMAX>(a, a.lvl_ptr, nlv, nbase, lane, nxt);
       } else {
-        csr_load_pass<FT, DMAX>(a, lv, base, lane, cur);
+        csr_load_pass<FT, DMAX>(a, a.lvl_ptr, lv, base, lane, cur);
       }
       if (cur.valid) {
         FT f = cur.hv;
@@ -162,8 +164,14 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   // logarithm; then the spins
   TT* s_t = reinterpret_cast<TT*>(smem);                                          // [4*ng]
   int8_t* s_s = reinterpret_cast<int8_t*>(smem + (size_t)ng * 4 * sizeof(TT));    // [4*ng]
+  // level pointers [nlev + 1]: read every pass, and global reads of them would
+  // be vector loads with a full wait (the barriers' memory clobbers keep the
+  // compiler from scalar loads)
+  int32_t* s_lp = reinterpret_cast<int32_t*>(smem + (size_t)ng * 4 * (sizeof(TT) + 1));
   int8_t* sp = a.spins + (int64_t)r * n;
   for (int i = tid; i < n; i += 64 * W) s_s[i] = sp[i] < 0 ? (int8_t)-1 : (int8_t)1;
+  for (int q = tid; q <= a.nlev; q += 64 * W) s_lp[q] = a.lvl_ptr[q];
+  __syncthreads();
 
   const uint32_t rep = (uint32_t)(a.rep_offset + r);
   const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   };
 
   CsrPass<FT, DMAX> cur;
-  csr_load_pass<FT, DMAX>(a, 0, 64 * w, lane, cur);
+  csr_load_pass<FT, DMAX>(a, s_lp, 0, 64 * w, lane, cur);
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
@@ -192,7 +200,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
     bar();  // thresholds of this sweep and spins of the last are visible
     int lv = 0, base = 64 * w;
     while (true) {
-      const int b1 = a.lvl_ptr[lv + 1];
+      const int b1 = a.lvl_ptr[__builtin_amdgcn_readfirstlane(lv) + 1];
       int nlv = lv, nbase = base + 64 * W;
       const bool end_of_level = nbase >= b1;
       bool last = false;
@@ -207,7 +215,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
         }
       }
       CsrPass<FT, DMAX> nxt;
-      csr_load_pass<FT, DMAX>(a, nlv, nbase, lane, nxt);
+      csr_load_pass<FT, DMAX>(a, s_lp, nlv, nbase, lane, nxt);
       if (cur.valid) {
         FT f = cur.hv;
 #pragma unroll
@@ -245,16 +253,18 @@ static hipError_t launch_csr_ft(const CsrArgs& a, dim3 grid, dim3 block, size_t 
   return hipGetLastError();
 }
 
-// LDS of the workgroup form: thresholds [4*ng] of `tbytes` each, then the
-// int8 spins [4*ng] (tbytes = sizeof(CsrThresh<FT>::type): 4 for FP32 fields,
-// 8 for FP64 and INT32 fields).
-static size_t csr_wg_lds_bytes(int n, size_t tbytes) {
-  return (size_t)((n + 3) >> 2) * (4 * tbytes + 4);
+// LDS of the workgroup form: thresholds [4*ng] of `tbytes` each, the int8
+// spins [4*ng] and the level pointers [nlev + 1] (tbytes =
+// sizeof(CsrThresh<FT>::type): 4 for FP32 fields, 8 for FP64 and INT32 fields).
+static size_t csr_wg_lds_bytes(int n, int nlev, size_t tbytes) {
+  return (size_t)((n + 3) >> 2) * (4 * tbytes + 4) + 4 * ((size_t)nlev + 1);
 }
+constexpr size_t CSR_WG_LDS_MAX = 160 * 1024;  // LDS of one gfx950 workgroup
 
 template <typename FT, int DMAX, int W>
 static hipError_t launch_csr_wg_d(const CsrArgs& a, hipStream_t st) {
-  const size_t lds = csr_wg_lds_bytes(a.n, sizeof(typename CsrThresh<FT>::type));
+  const size_t lds = csr_wg_lds_bytes(a.n, a.nlev, sizeof(typename CsrThresh<FT>::type));
+  if (lds > CSR_WG_LDS_MAX) return hipErrorInvalidValue;
   auto kern = k_csr_wg<FT, DMAX, W>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -280,17 +290,20 @@ static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
 
 // Waves per replica for the workgroup form (1 = the wave-per-replica kernel):
 // several waves split a level's passes, so each SIMD holds more waves and each
-// wave fewer serial passes.  `requested`
-// (1, 2, 4 or 8) overrides the choice; 0 chooses by R.
-int csr_waves_per_replica(int64_t R, int D, int requested) {
+// wave fewer serial passes.  `requested` (1, 2, 4 or 8) overrides the choice;
+// 0 chooses by R.  Graphs of degree > 16, or whose thresholds, spins and level
+// pointers exceed one workgroup's LDS, take the wave-per-replica kernel.
+int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int ftype) {
   if (D > 16) return 1;
+  const size_t tb = ftype == FT_F32 ? 4 : 8;
+  if (csr_wg_lds_bytes(n, nlev, tb) > CSR_WG_LDS_MAX) return 1;
   if (requested > 0) return requested;
   return R <= 512 ? 4 : 2;  // measured on 3D EA L=16: 2 waves beat 1 up to R = 4096 at least
 }
 
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hipStream_t st) {
   if (a.R <= 0 || a.T < 0) return hipSuccess;
-  const int W = csr_waves_per_replica(a.R, a.D, waves);
+  const int W = csr_waves_per_replica(a.R, a.D, waves, a.n, a.nlev, ftype);
   if (W > 1) {
     if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
     if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);

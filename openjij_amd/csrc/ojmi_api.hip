@@ -1248,7 +1248,7 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->lds_budget = c->lds_budget;
   info->num_problems = (c->kind == 2) ? c->K : 1;
   info->waves_per_replica =
-      (c->family == OJMI_KERNEL_CSR_LEVEL) ? csr_waves_per_replica(c->R, c->D, c->csr_waves) : 1;
+      (c->family == OJMI_KERNEL_CSR_LEVEL) ? csr_waves_per_replica(c->R, c->D, c->csr_waves, (int)c->n, c->nlev, c->ftype) : 1;
   info->reserved = 0;
   return OJMI_OK;
 }

@@ -112,8 +112,9 @@ hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W
 // ELL width of the CSR kernels for maximum degree D (8, 16, 32 or 64).
 int csr_padded_width(int D);
 // Waves per replica the CSR launcher uses (1 = one wave per replica, several
-// replicas per workgroup); `requested` 0 chooses by R and D.
-int csr_waves_per_replica(int64_t R, int D, int requested);
+// replicas per workgroup); `requested` 0 chooses by R; D, n, nlev and the
+// field type decide whether the workgroup form applies at all.
+int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int ftype);
 // waves_per_block: replicas per workgroup of the wave-per-replica form;
 // waves: 0 (choose), 1, 2, 4 or 8 waves per replica.
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, int waves,

@@ -134,6 +134,47 @@ def test_csr_waves_option_bitexact(waves):
     np.testing.assert_array_equal(got, want)
 
 
+def chain_csr(n, seed):
+    """Open 1D chain with integer couplings and fields: level(i) = i, so a
+    sweep has n levels (the most level pointers a graph can have)."""
+    w = H.splitmix64_np(seed, 2 * n)
+    mag = (w[:n] >> np.uint64(61)).astype(np.int64) + 1          # never zero: one chain
+    J = np.where((w[:n] >> np.uint64(60)) & np.uint64(1), mag, -mag).astype(float)
+    h = ((w[n:] >> np.uint64(61)).astype(np.int64) - 4).astype(float)
+    rowptr, col, val = [0], [], []
+    for i in range(n):
+        for j, v in ((i - 1, J[i - 1] if i > 0 else 0.0), (i + 1, J[i] if i < n - 1 else 0.0),
+                     (n, h[i])):
+            if j >= 0 and v != 0:
+                col.append(j)
+                val.append(v)
+        rowptr.append(len(col))
+    for i in range(n):
+        if h[i] != 0:
+            col.append(i)
+            val.append(h[i])
+    col.append(n)
+    val.append(1.0)
+    rowptr.append(len(col))
+    return np.array(rowptr, np.int64), np.array(col, np.int64), np.array(val, float)
+
+
+@pytest.mark.parametrize("n,waves", [(2000, 4), (13000, 1)])
+def test_csr_chain_many_levels(n, waves):
+    """n levels per sweep: the workgroup form keeps the level pointers in LDS
+    when they fit (n = 2000), the wave-per-replica form takes the rest."""
+    rp, ci, v = chain_csr(n, 0xC4A1 + n)
+    beta, mcs = sched(3, 0.3, 2.0)
+    init = H.random_pm1(4, n, 21)
+    with capi.Context.csr(n, rp, ci, v, 4, precision=capi.PREC_INT32) as ctx:
+        assert ctx.info()["waves_per_replica"] == waves
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 55, 0)
+        got = ctx.get_spins()
+    want = orc.replay_csr("int32", rp, ci, v, n, init, 55, 0, orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(got, want)
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_csr_real_couplings_bitexact(prec):
     E = H.sk_gauss(90, 4242, field=True)
