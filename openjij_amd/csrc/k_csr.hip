@@ -92,8 +92,8 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
-    if constexpr (sizeof(TT) == 8) nb = a.p0_d[t];
-    else nb = a.p0_f[t];
+    if constexpr (sizeof(TT) == 8) nb = sweep_scalar(a.p0_d[t]);
+    else nb = sweep_scalar(a.p0_f[t]);
     for (int g = lane; g < ng; g += 64) {
       uint32_t w0, w1, w2, w3;
       philox4x32_10((uint32_t)g, tg, rep, 0u, slo, shi, w0, w1, w2, w3);
@@ -187,8 +187,8 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
-    if constexpr (sizeof(TT) == 8) nb = a.p0_d[t];
-    else nb = a.p0_f[t];
+    if constexpr (sizeof(TT) == 8) nb = sweep_scalar(a.p0_d[t]);
+    else nb = sweep_scalar(a.p0_f[t]);
     for (int g = tid; g < ng; g += 64 * W) {
       uint32_t w0, w1, w2, w3;
       philox4x32_10((uint32_t)g, tg, rep, 0u, slo, shi, w0, w1, w2, w3);

@@ -91,7 +91,7 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
   constexpr int NQ = (NS + 3) / 4;
   for (int t = 0; t < T; ++t) {
     const uint32_t tg = (uint32_t)(t0 + (uint64_t)t);
-    const TT nb = p0[t];
+    const TT nb = sweep_scalar(p0[t]);
     for (int Q = 0; Q < NQ; ++Q) {
       uint32_t w[4];
       slot_words(slo, shi, tg, rep, slice, Q, lane, w);

@@ -109,7 +109,7 @@ __global__ __launch_bounds__(LDS_THREADS) void k_dense_lds(DenseArgs a, int Npad
 
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
-    const TT nb = p0[t];
+    const TT nb = sweep_scalar(p0[t]);
     for (int kb = 0; kb < NB; ++kb) {
       // ---- A: wave 0 decides block kb
       if (w == 0) {

@@ -300,8 +300,8 @@ __global__ __launch_bounds__(64 * G) void k_dense_shared(SharedArgs a) {
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
-    if constexpr (sizeof(TT) == 8) nb = a.p0_d[t];
-    else nb = a.p0_f[t];
+    if constexpr (sizeof(TT) == 8) nb = sweep_scalar(a.p0_d[t]);
+    else nb = sweep_scalar(a.p0_f[t]);
     for (int Q = 0; Q < NQ; ++Q) {
       uint32_t wd[4];
       {

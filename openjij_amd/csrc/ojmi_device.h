@@ -116,6 +116,20 @@ __device__ __forceinline__ float log_uniform<float>(uint32_t u) {
 // i.e. dE < log(U / (1 - U)) * (-1/beta)).  1 - U is exactly the uniform of
 // ~u in both precisions (U = (u + 1/2) 2^-32, or ((u >> 9) + 1/2) 2^-23), so
 // log(1 - U) is the same deterministic logarithm of another word.
+// A per-sweep parameter read from memory by every lane, moved to scalar
+// registers: the load is waited for once, where it is read, instead of at
+// every later use inside the sweep (where the wait would also drain
+// prefetches in flight).  All lanes hold the same value, so this is exact.
+__device__ __forceinline__ float sweep_scalar(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double sweep_scalar(double x) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 constexpr int RULE_METROPOLIS = 0;
 constexpr int RULE_HEAT_BATH = 1;
 template <typename TT>
