@@ -200,7 +200,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
     bar();  // thresholds of this sweep and spins of the last are visible
     int lv = 0, base = 64 * w;
     while (true) {
-      const int b1 = a.lvl_ptr[__builtin_amdgcn_readfirstlane(lv) + 1];
+      const int b1 = __builtin_amdgcn_readfirstlane(s_lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
       int nlv = lv, nbase = base + 64 * W;
       const bool end_of_level = nbase >= b1;
       bool last = false;

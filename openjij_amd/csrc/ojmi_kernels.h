@@ -28,7 +28,7 @@ struct DenseArgs {
   int rule;            // classical acceptance rule: RULE_METROPOLIS / RULE_HEAT_BATH
 };
 
-// Batch of independent dense classical problems (k_dense.hip, k_batch_wave):
+// Batch of independent dense classical problems (k_batch.hip, k_batch_wave):
 // per-problem offsets into concatenated buffers; `inst` lists the problems of
 // one launch (one slot class), each annealed as R replicas.
 struct BatchArgs {

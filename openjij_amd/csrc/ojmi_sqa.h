@@ -1,0 +1,188 @@
+// ojmi_sqa.h -- Suzuki-Trotter dense sweeps, one workgroup per replica: the
+// kernel template, instantiated by k_sqa_*.hip (one unit per field type and
+// slot range, so that they compile in parallel).
+#pragma once
+#include "ojmi_anneal.h"
+
+namespace ojmi {
+
+// ---------------------------------------------------------------------------
+// Transverse (Suzuki-Trotter): one workgroup per replica, wave w owns the
+// slice pairs p = w, w+W, ... (slices 2p and 2p+1), so every wave has one
+// slice in the even phase and one in the odd phase.  Phase order follows the
+// reference: even slices, odd slices, then slice M-1 when M is odd
+// (include/openjij/updater/single_spin_flip.hpp:160-192).  The spin words of
+// all slices live in LDS (one 64-bit ballot per slice and slot) so that each
+// lane can read the up/down neighbours of its own spin.
+//   dEcl = -2 s F,  dEtr = -2 s (s_up + s_dn),  dE = fma(a, dEcl, B*dEtr),
+//   flip iff dE < -log(U)  (a = s*beta/M, B = 1/2 log tanh(beta*Gamma*(1-s)/M)).
+template <typename FT, int NS, int PPW, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
+  typedef FT TT;
+  extern __shared__ uint64_t s_down[];  // [M][NS]
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int W = blockDim.x >> 6;
+  const int r = blockIdx.x;
+  const int n = a.n, M = a.M;
+  const int P = (M + 1) / 2;
+  const char* J = reinterpret_cast<const char*>(a.J);
+  constexpr int64_t RB = row_bytes<FT, NS>();
+  const FT* h = reinterpret_cast<const FT*>(a.h);
+
+  uint64_t down[2 * PPW];
+  FieldVec<FT, NS> F[2 * PPW];
+#pragma unroll
+  for (int pp = 0; pp < PPW; ++pp) {
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const int p = w + pp * W;
+      const int t = 2 * p + par;
+      down[2 * pp + par] = 0;
+      if (p < P && t < M) {
+        const int8_t* sp = a.spins + ((int64_t)r * M + t) * n;
+        uint64_t d = 0;
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+          const int j = 64 * m + lane;
+          if (j < n && sp[j] < 0) d |= 1ull << m;
+        }
+        down[2 * pp + par] = d;
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+          const uint64_t bal = ballot64((d >> m) & 1ull);
+          if (lane == 0) s_down[t * NS + m] = bal;
+        }
+        init_fields_seq<FT, FT, NS>(F[2 * pp + par], J, h, d, n, lane);
+      }
+    }
+  }
+  __syncthreads();
+
+  const uint32_t rep = (uint32_t)(a.rep_offset + r);
+  const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
+  constexpr int NQ = (NS + 3) / 4;
+  const int upper = (M & 1) ? M - 1 : M;
+
+  for (int tt = 0; tt < a.T; ++tt) {
+    const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)tt);
+    const FT at = sweep_param<TT>(a, tt);
+    const FT Bt = sweep_param1<TT>(a, tt);
+    // phase 0: even slices < upper; phase 1: odd slices; phase 2: slice M-1 (M odd)
+    for (int phase = 0; phase < 3; ++phase) {
+      if (phase == 2 && !(M & 1)) break;
+#pragma unroll
+      for (int pp = 0; pp < PPW; ++pp) {
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+          const int p = w + pp * W;
+          const int t = 2 * p + par;
+          bool act;
+          if (phase == 0) act = (par == 0) && (t < upper);
+          else if (phase == 1) act = (par == 1) && (t < M);
+          else act = (par == 0) && (t == M - 1);
+          if (p >= P || !act) continue;
+          const int tu = (t + 1) % M, td = (t + M - 1) % M;
+          FieldVec<FT, NS>& Ft = F[2 * pp + par];
+          uint64_t& dw = down[2 * pp + par];
+          for (int Q = 0; Q < NQ; ++Q) {
+            uint32_t wd[4];
+            slot_words(slo, shi, tg, rep, (uint32_t)t, Q, lane, wd);
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+              const int k = 4 * Q + jj;
+              if (k >= NS) break;
+              const int i = 64 * k + lane;
+              const TT th = (i < n) ? -log_uniform<TT>(wd[jj]) : neg_inf<TT>();
+              uint64_t live = ballot64(i < n);
+              uint64_t sb = ballot64((dw >> k) & 1ull);
+              const uint64_t bu = s_down[tu * NS + k], bd = s_down[td * NS + k];
+              // s_up + s_dn = 2 - 2(u + d)
+              const int nb2 = 2 - 2 * (int)(((bu >> lane) & 1ull) + ((bd >> lane) & 1ull));
+              while (true) {
+                const FT fk = Ft.dyn(k);
+                const bool dn = (sb >> lane) & 1ull;
+                const FT dEcl = dn ? (FT)2 * fk : (FT)(-2) * fk;
+                const FT dEtr = (FT)(dn ? 2 * nb2 : -2 * nb2);
+                const FT x = Bt * dEtr;
+                const FT dE = fma_ft<FT>(at, dEcl, x);
+                const uint64_t cand = ballot64(dE < th) & live;
+                if (cand == 0) break;
+                const int i0 = __builtin_ctzll(cand);
+                live &= lanes_above(i0);
+                const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
+                sb ^= 1ull << i0;
+                if (lane == i0) dw ^= 1ull << k;
+                row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
+              }
+              if (lane == 0) s_down[t * NS + k] = sb;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+#pragma unroll
+  for (int pp = 0; pp < PPW; ++pp) {
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const int p = w + pp * W;
+      const int t = 2 * p + par;
+      if (p < P && t < M) {
+        int8_t* sp = a.spins + ((int64_t)r * M + t) * n;
+#pragma unroll
+        for (int m = 0; m < NS; ++m) {
+          const int j = 64 * m + lane;
+          if (j < n) sp[j] = ((down[2 * pp + par] >> m) & 1ull) ? (int8_t)-1 : (int8_t)1;
+        }
+      }
+    }
+  }
+}
+
+template <typename FT, int NS>
+inline hipError_t launch_sqa_ns(const DenseArgs& a, int PPW, int W, hipStream_t st) {
+  dim3 grid(a.R), block(64 * W);
+  const size_t lds = (size_t)a.M * NS * sizeof(uint64_t);
+  // W > 4 runs the 1024-thread variant (<= 128 VGPRs), W <= 4 the 256-thread one.
+  if (W > 4) {
+    switch (PPW) {
+      case 1:
+        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 1, 1024>), grid, block, lds, st, a);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 2, 1024>), grid, block, lds, st, a);
+        break;
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else {
+    switch (PPW) {
+      case 1:
+        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 1, 256>), grid, block, lds, st, a);
+        break;
+      case 2:
+        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 2, 256>), grid, block, lds, st, a);
+        break;
+      case 4:
+        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 4, 256>), grid, block, lds, st, a);
+        break;
+      case 8:
+        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 8, 256>), grid, block, lds, st, a);
+        break;
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+// launch_sqa_<ft>_<range>: NS in {1, 2, 4, 8} (small) or {16, 32} (large)
+hipError_t launch_sqa_f32_small(const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+hipError_t launch_sqa_f32_large(const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+hipError_t launch_sqa_f64_small(const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+hipError_t launch_sqa_f64_large(const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+
+}  // namespace ojmi

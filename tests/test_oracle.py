@@ -394,3 +394,149 @@ def test_heat_bath_sweeps_sample_the_boltzmann_distribution():
         cnt = np.bincount(idx, minlength=2 ** n)
         stat = float((((cnt - R * p) ** 2) / (R * p)).sum())
         assert chi2.sf(stat, 2 ** n - 1) > 1e-4, (cnt, R * p)
+
+
+# --- Trotter replay pins --------------------------------------------------------
+def _trotter_states(n, M):
+    """All 2^(n M) Trotter configurations as [count][M][n] +-1 (bit m*n+i set = -1)."""
+    k = np.arange(2 ** (n * M), dtype=np.int64)[:, None]
+    bits = (k >> np.arange(n * M)) & 1
+    return (1 - 2 * bits).astype(np.int8).reshape(-1, M, n)
+
+
+def _trotter_hamiltonian(E_sym, states, a, B):
+    """Dimensionless Suzuki-Trotter weight exponent of the reference update
+    (include/openjij/updater/single_spin_flip.hpp:196-251): flipping s(i,t)
+    changes a * E_cl(slice t) by a * dE_cl and sum_t sum_i s(i,t) s(i,t+1)
+    by dE_trot, and the move is taken iff a dE_cl + B dE_trot < -log(U), so
+    the invariant law is exp(-H) with
+      H(S) = a sum_t E_cl(s_t) + B sum_t sum_i s(i,t) s(i,t+1 mod M)."""
+    M = states.shape[1]
+    ecl = np.array([[orc.energy_exact(E_sym, st[t]) for t in range(M)] for st in states])
+    ring = (states * np.roll(states, -1, axis=1)).sum(axis=(1, 2)).astype(np.float64)
+    return a * ecl.sum(axis=1) + B * ring
+
+
+def _trotter_chi2(out, p):
+    from scipy.stats import chi2
+
+    R, M, n = out.shape
+    idx = ((out.reshape(R, M * n) < 0).astype(np.int64) << np.arange(M * n)).sum(axis=1)
+    cnt = np.bincount(idx, minlength=p.size)
+    expect = R * p
+    stat = float((((cnt - expect) ** 2) / expect).sum())
+    return chi2.sf(stat, p.size - 1)
+
+
+@pytest.fixture(scope="module")
+def trotter_system():
+    n, M = 2, 4
+    Ju = np.zeros((n + 1, n + 1))
+    Ju[0, 1] = 0.8
+    Ju[0, n], Ju[1, n] = 0.3, -0.5
+    Ju[n, n] = 1.0
+    E = H.symmetric_ext(Ju)
+    beta, gamma, s = 2.0, 1.0, 0.5
+    T = 40
+    a, B = orc.expand_transverse(np.array([beta]), np.array([s]), np.array([T], np.int64),
+                                 gamma, M)
+    states = _trotter_states(n, M)
+    return dict(n=n, M=M, E=E, a=a, B=B, states=states, beta=beta, gamma=gamma, s=s)
+
+
+def test_expand_transverse_closed_form(trotter_system):
+    """a = s beta / M and B = 1/2 log tanh(beta Gamma (1 - s) / M) per sweep
+    (single_spin_flip.hpp:147-148, :216-217), repeated one_mc_step times."""
+    beta = np.array([0.5, 2.0, 7.0])
+    s = np.array([0.0, 0.3, 0.9])
+    mcs = np.array([2, 1, 3], np.int64)
+    for M in (2, 4, 7):
+        a, B = orc.expand_transverse(beta, s, mcs, 1.5, M)
+        ea = np.repeat(s * beta / M, mcs)
+        eB = np.repeat(0.5 * np.log(np.tanh(beta * 1.5 * (1.0 - s) / M)), mcs)
+        np.testing.assert_allclose(a, ea, rtol=1e-15)
+        np.testing.assert_allclose(B, eB, rtol=1e-14)
+        assert np.all(B < 0)  # tanh in (0, 1): ferromagnetic interslice coupling
+
+
+@pytest.mark.parametrize("which", ["dense", "csr"])
+def test_trotter_replay_samples_the_suzuki_trotter_distribution(trotter_system, which):
+    """Sequential Trotter sweeps leave exp(-H) invariant (each single-site
+    Metropolis step does): after 40 sweeps at fixed (beta, Gamma, s) from
+    independent random slices, 60000 replicas of a 2-spin x 4-slice system
+    match the enumerated law over its 256 states (chi-square, p > 1e-4).
+    The same test rejects the law of a wrong interslice sign (B -> -B) and of
+    the classical factor beta taken for s beta / M (p < 1e-12)."""
+    ts = trotter_system
+    n, M, E, a, B = ts["n"], ts["M"], ts["E"], ts["a"], ts["B"]
+    R = 60000
+    init = H.random_pm1(R * M, n, 0x7A07).reshape(R, M, n)
+    Hs = _trotter_hamiltonian(E, ts["states"], a[0], B[0])
+    p = np.exp(-(Hs - Hs.min()))
+    p /= p.sum()
+    rp, ci, v = orc.dense_to_csr(E)
+
+    def run(aa, BB):
+        if which == "dense":
+            return orc.replay_sqa_dense("fp64", E, init, 0x51A, 0, aa, BB)
+        return orc.replay_sqa_csr("fp64", rp, ci, v, n, init, 0x51A, 0, aa, BB)
+
+    assert _trotter_chi2(run(a, B), p) > 1e-4
+    # planted mistakes (in the replay's inputs): each changes the invariant law
+    assert _trotter_chi2(run(a, -B), p) < 1e-12
+    assert _trotter_chi2(run(np.full_like(a, ts["beta"]), B), p) < 1e-12
+
+
+def test_trotter_replay_odd_slice_count_stationary():
+    """M = 3 exercises the leftover-slice phase (single_spin_flip.hpp:184-191)."""
+    n, M, R = 2, 3, 50000
+    Ju = np.zeros((n + 1, n + 1))
+    Ju[0, 1], Ju[0, n], Ju[1, n], Ju[n, n] = -0.6, 0.2, 0.4, 1.0
+    E = H.symmetric_ext(Ju)
+    a, B = orc.expand_transverse(np.array([1.5]), np.array([0.4]), np.array([30], np.int64),
+                                 2.0, M)
+    states = _trotter_states(n, M)
+    Hs = _trotter_hamiltonian(E, states, a[0], B[0])
+    p = np.exp(-(Hs - Hs.min()))
+    p /= p.sum()
+    init = H.random_pm1(R * M, n, 0x3A3).reshape(R, M, n)
+    assert _trotter_chi2(orc.replay_sqa_dense("fp64", E, init, 77, 0, a, B), p) > 1e-4
+
+
+def test_trotter_replay_csr_equals_dense_on_integer_couplings():
+    """Integer couplings: recomputed (CSR) and incremental (dense) fields are
+    exact in FP64, so the two replays follow the same trajectory."""
+    n, M = 40, 6
+    E = H.symmetric_ext(H.int_dense(n, 0x7C5, density=0.3))
+    L = 25
+    a, B = orc.expand_transverse(np.full(L, 0.8), np.linspace(0.0, 0.9, L),
+                                 np.ones(L, np.int64), 3.0, M)
+    init = H.random_pm1(5 * M, n, 14).reshape(5, M, n)
+    rp, ci, v = orc.dense_to_csr(E)
+    d = orc.replay_sqa_dense("fp64", E, init, 901, 2, a, B)
+    c = orc.replay_sqa_csr("fp64", rp, ci, v, n, init, 901, 2, a, B)
+    assert not np.array_equal(d, init)
+    np.testing.assert_array_equal(d, c)
+
+
+def test_trotter_replay_matches_reference_restatement_ks():
+    """Two-sample KS of final per-read minimum slice energies: the GPU-rule
+    Trotter replay (Philox, threshold form) against the reference restatement
+    (xorshift + libstdc++ uniforms, rand_pool per sweep) at the same schedule.
+    SK N=20, M=4, Gamma=2, beta=3, s 0 -> 0.95 over 30 sweeps, 800 reads.
+    Threshold p > 1e-3."""
+    n, M, R, L = 20, 4, 800, 30
+    Eu = H.sk_gauss(n, 0x5A5)
+    E = H.symmetric_ext(Eu)
+    beta = np.full(L, 3.0)
+    s = np.linspace(0.0, 0.95, L)
+    mcs = np.ones(L, np.int64)
+    gamma = 2.0
+    init = H.random_pm1(R * M, n, 0x11).reshape(R, M, n)
+    a, B = orc.expand_transverse(beta, s, mcs, gamma, M)
+    rep = orc.replay_sqa_dense("fp64", E, init, 0xABC, 0, a, B)
+    ref = np.stack([orc.ref_sqa_dense(E, gamma, init[r], 1 + 3 * r, beta, s, mcs)
+                    for r in range(R)])
+    e_rep = np.array([min(orc.energy_exact(Eu, x) for x in rep[r]) for r in range(R)])
+    e_ref = np.array([min(orc.energy_exact(Eu, x) for x in ref[r]) for r in range(R)])
+    assert H.ks_2samp_pvalue(e_rep, e_ref) > 1e-3
