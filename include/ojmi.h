@@ -66,12 +66,12 @@ typedef struct ojmi_info {
   int64_t num_trotter;    /* M (1 for classical)                             */
   int32_t precision;      /* resolved OJMI_PREC_*                            */
   int32_t kernel;         /* OJMI_KERNEL_*                                   */
-  int32_t refresh_every;  /* sweeps between field refreshes (always 0: none) */
+  int32_t refresh_every;  /* sweeps between matrix-core field refreshes (0: none) */
   int32_t coupling_bytes; /* stored bytes per coupling (1, 4 or 8)           */
   int64_t last_sweeps;    /* expanded sweeps of the last ojmi_run            */
   int64_t last_attempts;  /* flip attempts of the last ojmi_run              */
   double last_kernel_ms;  /* device time of the last ojmi_run (HIP events)   */
-  double last_refresh_ms; /* part of it spent in field refreshes             */
+  double last_refresh_ms; /* part of it spent in matrix-core field refreshes */
   int32_t group;          /* replicas per workgroup (1 = one wave each)      */
   int32_t ring_rows;      /* coupling rows per LDS staging round (shared)    */
   int32_t lds_bytes;      /* LDS per workgroup of the last launch            */
@@ -79,6 +79,11 @@ typedef struct ojmi_info {
   int64_t num_problems;   /* problems of a batch context (1 otherwise)       */
   int32_t waves_per_replica; /* CSR: waves sharing one replica's sweep        */
   int32_t reserved;
+  int64_t last_refreshes; /* matrix-core field computations of the last run  */
+  int64_t last_flips;     /* accepted flips of the last run (counted on the GPU) */
+  int64_t last_rows;      /* coupling rows the last run streamed: each row read
+                             once per workgroup per block it was flipped in,
+                             (per replica for one-wave-per-replica kernels)  */
 } ojmi_info;
 
 /* Thread-local message of the last failing call on this thread. */
@@ -137,9 +142,13 @@ int ojmi_run_batch(ojmi_ctx* ctx, const double* beta, const int64_t* one_mc_step
 
 /*
  * Tuning knobs (no reference counterpart).  Keys:
- *   "refresh_every": sweeps between field refreshes; only 0 (fields are built
- *                    at the start of each run and updated per flip) is
- *                    accepted, other values return OJMI_ENOTSUP
+ *   "refresh_every": classical dense FP32 contexts: K > 0 computes the local
+ *                    fields on the matrix cores (ojmi_refresh_fields) at the
+ *                    start of each run and again every K sweeps of it, so that
+ *                    FP32 rounding never accumulates over more than K sweeps;
+ *                    0 (default) builds them in the sweep kernel and carries
+ *                    them through the run (the FP32 replay's arithmetic).
+ *                    Refresh points count from the start of each ojmi_run.
  *   "group":         replicas per workgroup sharing coupling rows (1, 4, 8, 16;
  *                    classical FP32 / INT32 dense systems)
  *   "xcd_window":    shared kernel: pace workgroups of one XCD to within this
@@ -192,8 +201,30 @@ int ojmi_run(ojmi_ctx* ctx, const double* beta, const double* s, const int64_t* 
  */
 int ojmi_solution(ojmi_ctx* ctx, int8_t* out);
 
-/* FP64 energies E(s) of every replica and slice, [R][M], evaluated on the GPU. */
+/* FP64 energies E(s) of every replica and slice, [R][M], evaluated on the GPU
+ * (H = S J in FP64 on the matrix cores, then E = sum_j s_j (H_j / 2 + h_j) in
+ * a fixed order): Dense::energy, include/openjij/graph/dense.hpp:176-192. */
 int ojmi_energies(ojmi_ctx* ctx, double* out);
+
+/*
+ * Local fields F = h + J s of every replica from its current spins, on the
+ * matrix cores, kept in the context (the fields the next sweep starts from
+ * when refresh_every > 0).  Replaces ClassicalIsing::reset_dE,
+ * include/openjij/system/classical_ising.hpp:87-90 (which stores
+ * dE = -2 s F), for all replicas at once.  FP32 contexts: J split into bf16
+ * J_hi + J_lo (|J - J_hi - J_lo| <= 2^-18 |J|), +-1 spins exact in bf16,
+ * FP32 accumulation.  INT32 contexts with |J| <= 127: int8 operands, exact
+ * int32 accumulation.  Other contexts return OJMI_ENOTSUP.
+ */
+int ojmi_refresh_fields(ojmi_ctx* ctx);
+
+/*
+ * The context's local fields, [R][n] as doubles (exact conversions of the
+ * FP32 or int32 values): after ojmi_refresh_fields, or after a run of an FP32
+ * context (the fields the sweep carried to its end).  The counterpart of
+ * reading ClassicalIsing::dE (classical_ising.hpp:111), as F = -dE / (2 s).
+ */
+int ojmi_get_fields(ojmi_ctx* ctx, double* out);
 
 /*
  * FP32 path: max over replicas and spins of |F32 - F64| between the fields the

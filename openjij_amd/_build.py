@@ -21,6 +21,7 @@ HIP_SOURCES = [
     "ojmi_api.hip", "k_wave.hip", "k_batch.hip", "k_sqa.hip", "k_sqa_f32_small.hip",
     "k_sqa_f32_large.hip", "k_sqa_f64_small.hip", "k_sqa_f64_large.hip", "k_shared.hip", "k_colown.hip",
     "k_colown_i8.hip", "k_colown_i32.hip", "k_lds.hip", "k_csr.hip", "k_energy.hip",
+    "k_spin_gemm.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -61,7 +62,7 @@ def build_engine(verbose: bool = False) -> str:
     with _cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp, *objs], verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-Wl,-z,defs", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
     return LIB
 

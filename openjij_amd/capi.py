@@ -30,6 +30,7 @@ class Info(C.Structure):
         ("last_refresh_ms", C.c_double), ("group", C.c_int32), ("ring_rows", C.c_int32),
         ("lds_bytes", C.c_int32), ("lds_budget", C.c_int32), ("num_problems", C.c_int64),
         ("waves_per_replica", C.c_int32), ("reserved", C.c_int32),
+        ("last_refreshes", C.c_int64), ("last_flips", C.c_int64), ("last_rows", C.c_int64),
     ]
 
 
@@ -66,6 +67,8 @@ def lib():
             "ojmi_destroy": (C.c_int, [vp]),
             "ojmi_create_batch": (C.c_int, [i64, vp, vp, i64, i32, C.POINTER(vp)]),
             "ojmi_run_batch": (C.c_int, [vp, vp, vp, i64, u64, u64]),
+            "ojmi_refresh_fields": (C.c_int, [vp]),
+            "ojmi_get_fields": (C.c_int, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -159,6 +162,15 @@ class Context:
         d = C.c_double(0)
         _check(lib().ojmi_field_drift(self.h, C.byref(d)))
         return d.value
+
+    def refresh_fields(self):
+        """Fields of every replica from its current spins on the matrix cores."""
+        _check(lib().ojmi_refresh_fields(self.h))
+
+    def get_fields(self):
+        o = np.empty((self.R, self.n), np.float64)
+        _check(lib().ojmi_get_fields(self.h, _p(o)))
+        return o
 
     def info(self) -> dict:
         i = Info()

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(256) void k_batch_wave(BatchArgs b) {
   int8_t* sp = b.spins + b.s_off[k] + (int64_t)r * n;
   const TT* p0 = p0_ptr<TT>(b.p0_d, b.p0_f) + (int64_t)k * b.T;
   anneal_replica<FT, JT, NS>(J, h, sp, n, nullptr, nullptr, (uint32_t)r, (uint32_t)k, b.seed,
-                             b.t0, b.T, p0, b.rule, lane);
+                             b.t0, b.T, p0, b.rule, lane, b.stats);
 }
 
 template <typename FT, typename JT>

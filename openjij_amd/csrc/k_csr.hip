@@ -89,6 +89,7 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
 
   CsrPass<FT, DMAX> cur;
   if constexpr (PREFETCH) csr_load_pass<FT, DMAX>(a, a.lvl_ptr, 0, 0, lane, cur);
+  uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
@@ -121,6 +122,7 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
       } else {
         csr_load_pass<FT, DMAX>(a, a.lvl_ptr, lv, base, lane, cur);
       }
+      bool flipped = false;
       if (cur.valid) {
         FT f = cur.hv;
 #pragma unroll
@@ -129,14 +131,17 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
         const int8_t si = s_s[i];
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
         const TT th = accept_log<TT>(s_w[i], a.rule) * nb;
-        if ((TT)dE < th) s_s[i] = (int8_t)-si;
+        flipped = (TT)dE < th;
+        if (flipped) s_s[i] = (int8_t)-si;
       }
+      nflip += (uint64_t)__builtin_popcountll(ballot64(flipped));
       if constexpr (PREFETCH) cur = nxt;
       if (last) break;
       lv = nlv;
       base = nbase;
     }
   }
+  if (a.stats && lane == 0) atomicAdd(a.stats, (unsigned long long)nflip);
   for (int i = lane; i < n; i += 64) sp[i] = s_s[i];
 }
 
@@ -184,6 +189,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
 
   CsrPass<FT, DMAX> cur;
   csr_load_pass<FT, DMAX>(a, s_lp, 0, 64 * w, lane, cur);
+  uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
@@ -216,6 +222,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
       }
       CsrPass<FT, DMAX> nxt;
       csr_load_pass<FT, DMAX>(a, s_lp, nlv, nbase, lane, nxt);
+      bool flipped = false;
       if (cur.valid) {
         FT f = cur.hv;
 #pragma unroll
@@ -223,8 +230,10 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
         const int i = cur.i;
         const int8_t si = s_s[i];
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
-        if ((TT)dE < s_t[i]) s_s[i] = (int8_t)-si;
+        flipped = (TT)dE < s_t[i];
+        if (flipped) s_s[i] = (int8_t)-si;
       }
+      nflip += (uint64_t)__builtin_popcountll(ballot64(flipped));
       cur = nxt;
       if (end_of_level) bar();  // this level's flips are visible to every wave
       if (last) break;
@@ -232,6 +241,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
       base = nbase;
     }
   }
+  if (a.stats && lane == 0) atomicAdd(a.stats, (unsigned long long)nflip);
   __syncthreads();
   for (int i = tid; i < n; i += 64 * W) sp[i] = s_s[i];
 }

@@ -72,13 +72,15 @@ __global__ __launch_bounds__(LDS_THREADS) void k_dense_lds(DenseArgs a, int Npad
   const FT* h = reinterpret_cast<const FT*>(a.h);
   int8_t* sp = a.spins + (int64_t)r * n;
 
+  const FT* F_in = a.F_in ? reinterpret_cast<const FT*>(a.F_in) + (int64_t)r * Npad : nullptr;
   for (int j = tid; j < Npad; j += LDS_THREADS) {
     sS[j] = (j < n) ? sp[j] : (int8_t)1;
-    sF[j] = h[j];
+    sF[j] = F_in ? F_in[j] : h[j];
   }
   __syncthreads();
-  // F = h + sum_k s_k J[k][.], k ascending, one rounding per term
-  for (int c0 = 0; c0 < Npad; c0 += LDS_THREADS * LDS_CH) {
+  // F = h + sum_k s_k J[k][.], k ascending, one rounding per term (unless the
+  // caller passed the fields)
+  for (int c0 = 0; c0 < (F_in ? 0 : Npad); c0 += LDS_THREADS * LDS_CH) {
     FT f[LDS_CH];
 #pragma unroll
     for (int m = 0; m < LDS_CH; ++m) {
@@ -107,6 +109,7 @@ __global__ __launch_bounds__(LDS_THREADS) void k_dense_lds(DenseArgs a, int Npad
   const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
   const TT* p0 = lds_p0<TT>(a);
 
+  uint64_t nflip = 0;  // workgroup-uniform; one row per flip (one replica per workgroup)
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     const TT nb = sweep_scalar(p0[t]);
@@ -152,6 +155,7 @@ __global__ __launch_bounds__(LDS_THREADS) void k_dense_lds(DenseArgs a, int Npad
       }
       __syncthreads();
       const uint64_t flips = sW[0], sb = sW[1];
+      nflip += (uint64_t)__builtin_popcountll(flips);
       // ---- B: rows of the flipped spins, ascending, onto every column
       if (flips) {
         const JT* Jb = J + (int64_t)(64 * kb) * Npad;
@@ -185,6 +189,10 @@ __global__ __launch_bounds__(LDS_THREADS) void k_dense_lds(DenseArgs a, int Npad
     }
   }
 
+  if (a.stats && tid == 0) {
+    atomicAdd(a.stats, (unsigned long long)nflip);
+    atomicAdd(a.stats + 1, (unsigned long long)nflip);
+  }
   for (int j = tid; j < n; j += LDS_THREADS) sp[j] = sS[j];
   if (a.F_out) {
     FT* fo = reinterpret_cast<FT*>(a.F_out) + (int64_t)r * Npad;

@@ -297,6 +297,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_shared(SharedArgs a) {
     wait_all_loads();
     __syncthreads();
   }
+  uint64_t nflip = 0, nrow = 0;  // wave-uniform: this wave's flips, the workgroup's rows
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
     TT nb;
@@ -351,6 +352,8 @@ __global__ __launch_bounds__(64 * G) void k_dense_shared(SharedArgs a) {
         uint64_t U = 0;
 #pragma unroll
         for (int g = 0; g < G; ++g) U |= sFlip[g];
+        nflip += (uint64_t)__builtin_popcountll(flips);
+        nrow += (uint64_t)__builtin_popcountll(U);
         // soft XCD pacing (performance only, bounded)
         if (a.xcd_window > 0 && w == 0 && done > (uint32_t)a.xcd_window) {
           const uint32_t need = done - (uint32_t)a.xcd_window;
@@ -395,6 +398,10 @@ __global__ __launch_bounds__(64 * G) void k_dense_shared(SharedArgs a) {
     }
   }
 
+  if (a.stats && lane == 0) {
+    atomicAdd(a.stats, (unsigned long long)nflip);
+    if (w == 0) atomicAdd(a.stats + 1, (unsigned long long)nrow);
+  }
   if (active) {
     int8_t* sp = a.spins + (int64_t)r * n;
 #pragma unroll

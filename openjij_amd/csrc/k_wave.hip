@@ -16,7 +16,7 @@ __global__ __launch_bounds__(256) void k_dense_wave(DenseArgs a) {
   anneal_replica<FT, JT, NS>(reinterpret_cast<const char*>(a.J), reinterpret_cast<const FT*>(a.h),
                              a.spins + (int64_t)r * a.n, a.n, F_in, F_out,
                              (uint32_t)(a.rep_offset + r), 0u, a.seed, a.t0, a.T,
-                             p0_ptr<TT>(a.p0_d, a.p0_f), a.rule, lane);
+                             p0_ptr<TT>(a.p0_d, a.p0_f), a.rule, lane, a.stats);
 }
 
 template <typename FT, typename JT>

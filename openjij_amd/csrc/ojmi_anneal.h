@@ -72,7 +72,8 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
                                                const FT* F_in, FT* F_out, uint32_t rep,
                                                uint32_t slice, uint64_t seed, uint64_t t0,
                                                int T, const typename ThreshT<FT>::type* p0,
-                                               int rule, int lane) {
+                                               int rule, int lane,
+                                               unsigned long long* stats) {
   typedef typename ThreshT<FT>::type TT;
   constexpr int64_t RB = row_bytes<JT, NS>();
   uint64_t down = 0;
@@ -91,6 +92,7 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
 
   const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
   constexpr int NQ = (NS + 3) / 4;
+  uint64_t nflip = 0;  // wave-uniform: one coupling row per accepted flip
   for (int t = 0; t < T; ++t) {
     const uint32_t tg = (uint32_t)(t0 + (uint64_t)t);
     const TT nb = sweep_scalar(p0[t]);
@@ -121,8 +123,13 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
           row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c);
         }
         down ^= ((flips >> lane) & 1ull) << k;
+        nflip += (uint64_t)__builtin_popcountll(flips);
       }
     }
+  }
+  if (stats && lane == 0) {
+    atomicAdd(stats, (unsigned long long)nflip);
+    atomicAdd(stats + 1, (unsigned long long)nflip);
   }
 
 #pragma unroll

@@ -44,6 +44,20 @@ int dalloc(T** p, size_t count) {
   return OJMI_OK;
 }
 
+// bfloat16 of a float, round to nearest even (finite inputs), and its value.
+uint16_t bf16_rn(float x) {
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+float bf16_value(uint16_t v) {
+  const uint32_t b = (uint32_t)v << 16;
+  float x;
+  std::memcpy(&x, &b, 4);
+  return x;
+}
+
 int pow2_slots(int64_t n) {
   int64_t need = (n + 63) / 64;
   int ns = 1;
@@ -87,11 +101,22 @@ struct ojmi_ctx {
   int nlev = 0, D = 0;
 
   int8_t* dspins = nullptr;
-  double* dH = nullptr;    // [C][Npad] FP64 fields for energies
   double* dE = nullptr;    // [C]
   double* ddrift = nullptr;
-  void* dF = nullptr;      // [R][Npad] carried fields (FP32 path)
-  bool fields_valid = false;
+  void* dF = nullptr;      // [R][Npad] fields (FP32 or int32): MFMA init / refresh, sweep output
+  bool fields_valid = false;  // dF holds the fields of the current spins (FP32 path)
+  bool energies_valid = false;  // dE / ddrift hold the current spins' energies and drift
+  // matrix-core operands: bf16 hi/lo couplings (FP32 path), spins zero-padded
+  // to [C][Npad] as int8 and bf16, energy partials
+  uint16_t* dJhi = nullptr;
+  uint16_t* dJlo = nullptr;
+  int8_t* dS8 = nullptr;
+  uint16_t* dSbf = nullptr;
+  double* dpart = nullptr;  // 2 x energy_part_count(C, Npad)
+  std::vector<hipEvent_t> ev_ref;  // refresh begin/end pairs of the last run
+  int64_t last_refreshes = 0;
+  unsigned long long* dstats = nullptr;  // [2] accepted flips, coupling rows streamed
+  int64_t last_flips = 0, last_rows = 0;
 
   double *dp0_d = nullptr, *dp1_d = nullptr;
   float *dp0_f = nullptr, *dp1_f = nullptr;
@@ -118,12 +143,14 @@ struct ojmi_ctx {
     if (dJ == dJ64) dJ = nullptr;
     if (dh == dh64) dh = nullptr;
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
-                    dH, dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
-                    d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f};
+                    dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
+                    d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
+                    dJhi, dJlo, dS8, dSbf, dpart, dstats};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     for (auto& cl : bclass)
       if (cl.second) (void)hipFree(cl.second);
+    for (hipEvent_t e : ev_ref) (void)hipEventDestroy(e);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
@@ -232,6 +259,18 @@ int upload_dense(ojmi_ctx* c, const std::vector<double>& Jn /* n x n symmetric *
     c->dh = dh;
     OJMI_HIP(hipMemcpy(dJ, Jf.data(), Jf.size() * 4, hipMemcpyHostToDevice));
     OJMI_HIP(hipMemcpy(dh, hf.data(), hf.size() * 4, hipMemcpyHostToDevice));
+    if (c->M == 1) {
+      // matrix-core operands of the field init / refresh: J = J_hi + J_lo in bf16
+      std::vector<uint16_t> hi(Jf.size()), lo(Jf.size());
+      for (size_t q = 0; q < Jf.size(); ++q) {
+        hi[q] = bf16_rn(Jf[q]);
+        lo[q] = bf16_rn(Jf[q] - bf16_value(hi[q]));
+      }
+      if ((rc = dalloc(&c->dJhi, hi.size())) != OJMI_OK) return rc;
+      if ((rc = dalloc(&c->dJlo, lo.size())) != OJMI_OK) return rc;
+      OJMI_HIP(hipMemcpy(c->dJhi, hi.data(), hi.size() * 2, hipMemcpyHostToDevice));
+      OJMI_HIP(hipMemcpy(c->dJlo, lo.data(), lo.size() * 2, hipMemcpyHostToDevice));
+    }
     return OJMI_OK;
   }
   // integer paths: h as int32
@@ -373,6 +412,9 @@ static int64_t batch_row_bytes(int jtype, int NS) {
   return (int64_t)NS * 64 * (jtype == JT_F64 ? 8 : 4);
 }
 
+int begin_stats(ojmi_ctx* c);
+int end_stats(ojmi_ctx* c);
+
 static int run_batch(ojmi_ctx* c, const std::vector<double>& p0 /* [K][T] */, int64_t T,
                      uint64_t seed, uint64_t sweep_offset) {
   c->last_T = T;
@@ -381,6 +423,8 @@ static int run_batch(ojmi_ctx* c, const std::vector<double>& p0 /* [K][T] */, in
   c->last_attempts = T * c->R * sum_n;
   c->last_ms = 0;
   c->last_refresh_ms = 0;
+  c->last_refreshes = 0;
+  c->last_flips = c->last_rows = 0;
   if (T == 0) return OJMI_OK;
   int rc;
   if (T > c->bTcap) {
@@ -412,6 +456,8 @@ static int run_batch(ojmi_ctx* c, const std::vector<double>& p0 /* [K][T] */, in
   b.p0_d = c->db_p0_d;
   b.p0_f = c->db_p0_f;
   b.rule = c->rule;
+  if ((rc = begin_stats(c))) return rc;
+  b.stats = c->dstats;
   OJMI_HIP(hipEventRecord(c->e0, c->st));
   // largest slot class first: the longest waves start first
   for (size_t q = c->bclass.size(); q-- > 0;) {
@@ -426,7 +472,7 @@ static int run_batch(ojmi_ctx* c, const std::vector<double>& p0 /* [K][T] */, in
   float ms = 0;
   OJMI_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
   c->last_ms = ms;
-  return OJMI_OK;
+  return end_stats(c);
 }
 
 static int batch_energies(ojmi_ctx* c) {
@@ -443,6 +489,191 @@ static int batch_energies(ojmi_ctx* c) {
   if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("batch energy: ") + hipGetErrorString(e));
   OJMI_HIP(hipStreamSynchronize(c->st));
   return OJMI_OK;
+}
+
+// Device counters of a run (accepted flips, coupling rows streamed), zeroed on
+// the context's stream before its first launch.
+int begin_stats(ojmi_ctx* c) {
+  int rc;
+  if (!c->dstats && (rc = dalloc(&c->dstats, 2))) return rc;
+  OJMI_HIP(hipMemsetAsync(c->dstats, 0, 2 * sizeof(unsigned long long), c->st));
+  return OJMI_OK;
+}
+// After the run's final synchronisation.
+int end_stats(ojmi_ctx* c) {
+  unsigned long long v[2] = {0, 0};
+  OJMI_HIP(hipMemcpy(v, c->dstats, sizeof(v), hipMemcpyDeviceToHost));
+  c->last_flips = (int64_t)v[0];
+  c->last_rows = (int64_t)v[1];
+  return OJMI_OK;
+}
+
+// [R][Npad] field buffer (4-byte elements: FP32 or int32 fields).
+int ensure_fields(ojmi_ctx* c) {
+  if (c->dF) return OJMI_OK;
+  float* f;
+  int rc = dalloc(&f, (size_t)(c->R * c->Npad));
+  if (rc) return rc;
+  c->dF = f;
+  return OJMI_OK;
+}
+
+// Zero-padded spins for the matrix-core kernels: int8 [R*M][Npad] (fields of
+// INT32 contexts, energies of every context) and bf16 [R][Npad] (FP32 fields).
+int ensure_mfma_spins(ojmi_ctx* c) {
+  int rc;
+  if (!c->dS8 && (rc = dalloc(&c->dS8, (size_t)(c->R * c->M * c->Npad)))) return rc;
+  if (c->ftype == FT_F32 && c->M == 1 && !c->dSbf &&
+      (rc = dalloc(&c->dSbf, (size_t)(c->R * c->Npad))))
+    return rc;
+  return OJMI_OK;
+}
+
+// XCD pacing counters of the workgroup kernels, one per workgroup.
+int ensure_progress(ojmi_ctx* c) {
+  const int64_t nwg = (c->R + 3) / 4;  // the most workgroups any group size launches
+  if (c->prog_cap >= nwg) return OJMI_OK;
+  if (c->dprog) (void)hipFree(c->dprog);
+  c->dprog = nullptr;
+  c->prog_cap = 0;
+  int rc = dalloc(&c->dprog, (size_t)nwg);
+  if (rc) return rc;
+  c->prog_cap = nwg;
+  return OJMI_OK;
+}
+
+// Fields of every replica from the current spins on the matrix cores, into
+// dF: bf16 hi/lo with FP32 accumulation (FP32 contexts) or int8 with exact
+// int32 accumulation (INT32 contexts with int8 couplings).
+hipError_t mfma_fields(ojmi_ctx* c) {
+  if (ensure_mfma_spins(c) != OJMI_OK) return hipErrorOutOfMemory;
+  const int R = (int)c->R, n = (int)c->n, Np = (int)c->Npad;
+  hipError_t e;
+  if (c->ftype == FT_F32) {
+    e = launch_pack_spins(c->dspins, n, R, Np, nullptr, c->dSbf, c->st);
+    if (e != hipSuccess) return e;
+    return launch_fields_bf16x2(c->dSbf, c->dJhi, c->dJlo, (const float*)c->dh, R, Np,
+                                (float*)c->dF, c->st);
+  }
+  e = launch_pack_spins(c->dspins, n, R, Np, c->dS8, nullptr, c->st);
+  if (e != hipSuccess) return e;
+  return launch_fields_i8(c->dS8, (const int8_t*)c->dJnat, (const int32_t*)c->dh, R, Np,
+                          (int32_t*)c->dF, c->st);
+}
+
+// One launch of the context's sweep kernel over the expanded sweeps
+// [t_begin, t_begin + T): global sweep indices sweep_offset + t_begin + t.
+// F_in: initial fields ([R][Npad], the field type) or null (the kernel builds
+// them in sweep order).  FP32 contexts write their final fields to dF.
+hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
+                         uint64_t sweep_offset, const void* F_in) {
+  if (c->family == OJMI_KERNEL_CSR_LEVEL) {
+    CsrArgs a{};
+    a.order = c->d_order;
+    a.lvl_ptr = c->d_lvl;
+    a.nlev = c->nlev;
+    a.deg = c->d_deg;
+    a.ell_col = c->d_ell;
+    a.val = c->d_ellval;
+    a.D = c->D;
+    a.h = c->dh;
+    a.spins = c->dspins;
+    a.n = (int)c->n;
+    a.R = (int)c->R;
+    a.M = 1;
+    a.T = (int)T;
+    a.seed = seed;
+    a.t0 = sweep_offset + (uint64_t)t_begin;
+    a.rep_offset = 0;
+    a.p0_d = c->dp0_d + t_begin;
+    a.p0_f = c->dp0_f + t_begin;
+    a.rule = c->rule;
+    a.stats = c->dstats;
+    return launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
+  }
+  void* F_out = (c->ftype == FT_F32 && c->M == 1) ? c->dF : nullptr;
+  DenseArgs a{};
+  a.J = c->dJ;
+  a.h = c->dh;
+  a.spins = c->dspins;
+  a.n = (int)c->n;
+  a.R = (int)c->R;
+  a.M = (int)c->M;
+  a.T = (int)T;
+  a.seed = seed;
+  a.t0 = sweep_offset + (uint64_t)t_begin;
+  a.rep_offset = 0;
+  a.p0_d = c->dp0_d + t_begin;
+  a.p0_f = c->dp0_f + t_begin;
+  a.p1_d = c->dp1_d + t_begin;
+  a.p1_f = c->dp1_f + t_begin;
+  a.rule = c->rule;
+  a.F_in = F_in;
+  a.F_out = F_out;
+  a.stats = c->dstats;
+  if (c->family == OJMI_KERNEL_SQA_WAVE) {
+    a.F_in = a.F_out = nullptr;
+    return launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
+  }
+  if (c->family == OJMI_KERNEL_DENSE_LDS) {
+    if (c->jtype == JT_I8) a.J = c->dJnat;
+    c->lds_used = (int)lds_kernel_bytes((int)c->Npad, c->ftype == FT_F64 ? 8 : 4);
+    return launch_dense_lds(c->ftype, c->jtype, a, (int)c->Npad, c->st);
+  }
+  if (c->family == OJMI_KERNEL_DENSE_WAVE) return launch_dense_wave(c->ftype, c->jtype, a, c->NS, c->st);
+
+  // workgroup-shared families
+  SharedArgs sa{};
+  sa.J = c->dJnat ? c->dJnat : c->dJ;
+  sa.h = c->dh;
+  sa.spins = c->dspins;
+  sa.n = (int)c->n;
+  sa.R = (int)c->R;
+  sa.T = (int)T;
+  sa.seed = seed;
+  sa.t0 = sweep_offset + (uint64_t)t_begin;
+  sa.rule = c->rule;
+  sa.rep_offset = 0;
+  sa.p0_d = c->dp0_d + t_begin;
+  sa.p0_f = c->dp0_f + t_begin;
+  sa.F_in = F_in;
+  sa.F_out = F_out;
+  sa.stats = c->dstats;
+  // heat-bath colown kernels exist for G = 4 only; a replica's trajectory
+  // does not depend on G, so such runs take G = 4 (grid and pacing below too)
+  const bool hb_colown = c->rule == RULE_HEAT_BATH && c->colown &&
+                         colown_supported((int)c->NS, c->group) &&
+                         colown_supported((int)c->NS, 4);
+  const int G = hb_colown ? 4 : c->group;
+  const int jb = (c->jtype == JT_I8) ? 1 : 4;
+  const bool use_colown = c->colown && colown_supported((int)c->NS, G);
+  const size_t fixed = shared_lds_bytes((int)c->NS, G, 0, jb);
+  const size_t per_row = (size_t)jb * 64 * c->NS;
+  int rb = (int)std::min<size_t>(64, ((size_t)c->lds_budget - fixed) / per_row);
+  if (c->ring_rows_opt > 0) rb = std::min(rb, c->ring_rows_opt);
+  // rounds of S rows, at least two buffers (one consumed, others in flight)
+  const int S = std::max(1, std::min(c->round_rows_opt > 0 ? c->round_rows_opt : 2, rb / 2));
+  rb = (rb / S) * S;
+  if (rb < 2 * S) return hipErrorInvalidValue;
+  sa.ring_rows = rb;
+  sa.round_rows = S;
+  c->ring_rows_used = rb;
+  c->lds_used = (int)shared_lds_bytes((int)c->NS, G, rb, jb);
+  const int64_t nwg = (c->R + G - 1) / G;
+  sa.xcd_window = (nwg >= 16) ? c->xcd_window : 0;
+  sa.xcd_spin = 4000;
+  if (sa.xcd_window > 0) {
+    if (c->prog_cap < nwg) return hipErrorInvalidValue;
+    hipError_t e = hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st);
+    if (e != hipSuccess) return e;
+    sa.progress = c->dprog;
+  }
+  if (use_colown) {
+    c->ring_rows_used = 0;
+    c->lds_used = (int)colown_lds_bytes((int)c->n, (int)c->NS, G, jb, 4);
+    return launch_dense_colown(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
+  }
+  return launch_dense_shared(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
 }
 
 }  // namespace
@@ -882,11 +1113,14 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   if (!c || !key) return fail(OJMI_EINVAL, "null argument");
   const std::string k(key);
   if (k == "refresh_every") {
-    if (value < 0) return fail(OJMI_EINVAL, "refresh_every must be >= 0");
-    // FP32 fields take one rounding per accepted flip in sweep order (replayed
-    // exactly by the CPU replay); no kernel recomputes them mid-run.
-    if (value > 0) return fail(OJMI_ENOTSUP, "periodic field refresh is not implemented");
-    c->refresh_every = 0;
+    if (value < 0 || value > INT32_MAX) return fail(OJMI_EINVAL, "refresh_every must be >= 0");
+    // FP32 dense classical contexts: fields from the matrix cores (bf16 hi/lo
+    // couplings, FP32 accumulation) at the start of each run and every
+    // `value` sweeps of it.  0: the sweep kernel builds them in sweep order and
+    // carries them through the run (the FP32 replay's arithmetic).
+    if (value > 0 && !(c->kind == 0 && c->M == 1 && c->ftype == FT_F32 && c->dJhi))
+      return fail(OJMI_ENOTSUP, "field refresh applies to classical dense FP32 contexts");
+    c->refresh_every = (int)value;
     return OJMI_OK;
   }
   if (k == "group") {
@@ -950,6 +1184,7 @@ int ojmi_set_spins(ojmi_ctx* c, const int8_t* spins) {
       return fail(OJMI_EINVAL, "spins must be +1 or -1");
   OJMI_HIP(hipMemcpy(c->dspins, spins, cnt, hipMemcpyHostToDevice));
   c->fields_valid = false;
+  c->energies_valid = false;
   return OJMI_OK;
 }
 
@@ -990,6 +1225,8 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
   c->last_attempts = T * c->R * c->M * c->n;
   c->last_ms = 0;
   c->last_refresh_ms = 0;
+  c->last_refreshes = 0;
+  c->last_flips = c->last_rows = 0;
   if (T == 0) return OJMI_OK;
   int rc;
   if ((rc = ensure_sched(c, T))) return rc;
@@ -1021,139 +1258,48 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
   OJMI_HIP(hipMemcpyAsync(c->dp1_d, p1.data(), T * 8, hipMemcpyHostToDevice, c->st));
   OJMI_HIP(hipMemcpyAsync(c->dp0_f, p0f.data(), T * 4, hipMemcpyHostToDevice, c->st));
   OJMI_HIP(hipMemcpyAsync(c->dp1_f, p1f.data(), T * 4, hipMemcpyHostToDevice, c->st));
+  if (c->family != OJMI_KERNEL_CSR_LEVEL && c->family != OJMI_KERNEL_SQA_WAVE &&
+      (rc = ensure_fields(c)))
+    return rc;
+  if ((c->family == OJMI_KERNEL_DENSE_SHARED) && (rc = ensure_progress(c))) return rc;
+  c->energies_valid = false;
+  if ((rc = begin_stats(c))) return rc;
 
   OJMI_HIP(hipEventRecord(c->e0, c->st));
   hipError_t e = hipSuccess;
-  if (c->family == OJMI_KERNEL_CSR_LEVEL) {
-    CsrArgs a{};
-    a.order = c->d_order;
-    a.lvl_ptr = c->d_lvl;
-    a.nlev = c->nlev;
-    a.deg = c->d_deg;
-    a.ell_col = c->d_ell;
-    a.val = c->d_ellval;
-    a.D = c->D;
-    a.h = c->dh;
-    a.spins = c->dspins;
-    a.n = (int)c->n;
-    a.R = (int)c->R;
-    a.M = 1;
-    a.T = (int)T;
-    a.seed = seed;
-    a.t0 = sweep_offset;
-    a.rep_offset = 0;
-    a.p0_d = c->dp0_d;
-    a.p0_f = c->dp0_f;
-    a.rule = c->rule;
-    e = launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
+  if (c->family == OJMI_KERNEL_CSR_LEVEL || c->family == OJMI_KERNEL_SQA_WAVE) {
+    e = launch_sweeps(c, 0, T, seed, sweep_offset, nullptr);
   } else {
-    DenseArgs a{};
-    a.J = c->dJ;
-    a.h = c->dh;
-    a.spins = c->dspins;
-    a.n = (int)c->n;
-    a.R = (int)c->R;
-    a.M = (int)c->M;
-    a.T = (int)T;
-    a.seed = seed;
-    a.t0 = sweep_offset;
-    a.rep_offset = 0;
-    a.p0_d = c->dp0_d;
-    a.p0_f = c->dp0_f;
-    a.p1_d = c->dp1_d;
-    a.p1_f = c->dp1_f;
-    a.rule = c->rule;
-    if (c->family == OJMI_KERNEL_SQA_WAVE) {
-      e = launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
-    } else if (c->family == OJMI_KERNEL_DENSE_LDS) {
-      if (c->jtype == JT_I8) a.J = c->dJnat;
-      if (c->ftype == FT_F32) {
-        if (!c->dF) {
-          float* f;
-          if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
-          c->dF = f;
+    // Fields: from the matrix cores (exact int8 for INT32 couplings; bf16
+    // hi/lo for FP32 when a refresh period is set, refreshed every
+    // refresh_every sweeps), else built by the sweep kernel itself in sweep
+    // order (the FP32 replay's order).
+    const bool mfma_i8 = c->ftype == FT_I32 && c->dJnat && c->M == 1;
+    const bool mfma_bf = c->ftype == FT_F32 && c->dJhi && c->refresh_every > 0;
+    const int64_t chunk = mfma_bf ? c->refresh_every : T;
+    for (int64_t t = 0; t < T && e == hipSuccess; t += chunk) {
+      const int64_t Tc = std::min<int64_t>(chunk, T - t);
+      const void* F_in = nullptr;
+      if (mfma_i8 || mfma_bf) {
+        if (mfma_bf) {
+          if ((size_t)(2 * c->last_refreshes + 2) > c->ev_ref.size()) {
+            hipEvent_t ev[2];
+            OJMI_HIP(hipEventCreate(&ev[0]));
+            OJMI_HIP(hipEventCreate(&ev[1]));
+            c->ev_ref.push_back(ev[0]);
+            c->ev_ref.push_back(ev[1]);
+          }
+          OJMI_HIP(hipEventRecord(c->ev_ref[2 * c->last_refreshes], c->st));
         }
-        a.F_out = c->dF;
+        e = mfma_fields(c);
+        if (mfma_bf) OJMI_HIP(hipEventRecord(c->ev_ref[2 * c->last_refreshes + 1], c->st));
+        if (mfma_bf) ++c->last_refreshes;
+        if (e != hipSuccess) break;
+        F_in = c->dF;
       }
-      c->lds_used = (int)lds_kernel_bytes((int)c->Npad, c->ftype == FT_F64 ? 8 : 4);
-      e = launch_dense_lds(c->ftype, c->jtype, a, (int)c->Npad, c->st);
-      c->fields_valid = (c->ftype == FT_F32);
-    } else if (c->family == OJMI_KERNEL_DENSE_SHARED) {
-      SharedArgs sa{};
-      sa.J = c->dJnat ? c->dJnat : c->dJ;
-      sa.h = c->dh;
-      sa.spins = c->dspins;
-      sa.n = (int)c->n;
-      sa.R = (int)c->R;
-      sa.T = (int)T;
-      sa.seed = seed;
-      sa.t0 = sweep_offset;
-      sa.rule = c->rule;
-      sa.rep_offset = 0;
-      sa.p0_d = c->dp0_d;
-      sa.p0_f = c->dp0_f;
-      if (c->ftype == FT_F32) {
-        if (!c->dF) {
-          float* f;
-          if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
-          c->dF = f;
-        }
-        sa.F_out = c->dF;
-      }
-      // heat-bath colown kernels exist for G = 4 only; a replica's trajectory
-      // does not depend on G, so such runs take G = 4 (grid and pacing below too)
-      const bool hb_colown = c->rule == RULE_HEAT_BATH && c->colown &&
-                             colown_supported((int)c->NS, c->group) &&
-                             colown_supported((int)c->NS, 4);
-      const int G = hb_colown ? 4 : c->group;
-      const int jb = (c->jtype == JT_I8) ? 1 : 4;
-      const bool use_colown = c->colown && colown_supported((int)c->NS, G);
-      const size_t fixed = shared_lds_bytes((int)c->NS, G, 0, jb);
-      const size_t per_row = (size_t)jb * 64 * c->NS;
-      int rb = (int)std::min<size_t>(64, ((size_t)c->lds_budget - fixed) / per_row);
-      if (c->ring_rows_opt > 0) rb = std::min(rb, c->ring_rows_opt);
-      // rounds of S rows, at least two buffers (one consumed, others in flight)
-      const int S = std::max(1, std::min(c->round_rows_opt > 0 ? c->round_rows_opt : 2, rb / 2));
-      rb = (rb / S) * S;
-      if (rb < 2 * S) return fail(OJMI_ENOTSUP, "shared kernel: LDS too small for two rows");
-      sa.ring_rows = rb;
-      sa.round_rows = S;
-      c->ring_rows_used = rb;
-      c->lds_used = (int)shared_lds_bytes((int)c->NS, G, rb, jb);
-      const int64_t nwg = (c->R + G - 1) / G;
-      sa.xcd_window = (nwg >= 16) ? c->xcd_window : 0;
-      sa.xcd_spin = 4000;
-      if (sa.xcd_window > 0) {
-        if (c->prog_cap < nwg) {
-          if (c->dprog) (void)hipFree(c->dprog);
-          c->dprog = nullptr;
-          if ((rc = dalloc(&c->dprog, (size_t)nwg))) return rc;
-          c->prog_cap = nwg;
-        }
-        OJMI_HIP(hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st));
-        sa.progress = c->dprog;
-        OJMI_HIP(hipEventRecord(c->e0, c->st));  // time the sweep kernel only
-      }
-      if (use_colown) {
-        c->ring_rows_used = 0;
-        c->lds_used = (int)colown_lds_bytes((int)c->n, (int)c->NS, G, jb, 4);
-        e = launch_dense_colown(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
-      } else {
-        e = launch_dense_shared(c->ftype, c->jtype, sa, (int)c->NS, G, c->st);
-      }
-      c->fields_valid = (c->ftype == FT_F32);
-    } else {
-      if (c->ftype == FT_F32) {
-        if (!c->dF) {
-          float* f;
-          if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
-          c->dF = f;
-        }
-        a.F_out = c->dF;
-      }
-      e = launch_dense_wave(c->ftype, c->jtype, a, c->NS, c->st);
-      c->fields_valid = (c->ftype == FT_F32);
+      e = launch_sweeps(c, t, Tc, seed, sweep_offset, F_in);
     }
+    c->fields_valid = (c->ftype == FT_F32);
   }
   if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   OJMI_HIP(hipEventRecord(c->e1, c->st));
@@ -1161,19 +1307,69 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
   float ms = 0;
   OJMI_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
   c->last_ms = ms;
+  for (int64_t q = 0; q < c->last_refreshes; ++q) {
+    float rm = 0;
+    OJMI_HIP(hipEventElapsedTime(&rm, c->ev_ref[2 * q], c->ev_ref[2 * q + 1]));
+    c->last_refresh_ms += rm;
+  }
+  return end_stats(c);
+}
+
+int ojmi_refresh_fields(ojmi_ctx* c) {
+  if (!c) return fail(OJMI_EINVAL, "null context");
+  const bool ok = c->kind == 0 && c->M == 1 &&
+                  ((c->ftype == FT_F32 && c->dJhi) || (c->ftype == FT_I32 && c->dJnat));
+  if (!ok)
+    return fail(OJMI_ENOTSUP,
+                "matrix-core fields need a classical dense FP32 or int8-coupling INT32 context");
+  int rc;
+  if ((rc = ensure_fields(c))) return rc;
+  hipError_t e = mfma_fields(c);
+  if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("field kernel: ") + hipGetErrorString(e));
+  OJMI_HIP(hipStreamSynchronize(c->st));
+  c->fields_valid = (c->ftype == FT_F32);
+  c->energies_valid = false;
+  return OJMI_OK;
+}
+
+int ojmi_get_fields(ojmi_ctx* c, double* out) {
+  if (!c || !out) return fail(OJMI_EINVAL, "null argument");
+  if (!c->dF || c->M != 1 || c->kind != 0)
+    return fail(OJMI_EINVAL, "no fields held: run or refresh a classical dense context first");
+  const int64_t n = c->n, R = c->R, Np = c->Npad;
+  OJMI_HIP(hipStreamSynchronize(c->st));
+  if (c->ftype == FT_F32) {
+    std::vector<float> f((size_t)(R * Np));
+    OJMI_HIP(hipMemcpy(f.data(), c->dF, f.size() * 4, hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < R; ++r)
+      for (int64_t j = 0; j < n; ++j) out[r * n + j] = (double)f[(size_t)(r * Np + j)];
+  } else {
+    std::vector<int32_t> f((size_t)(R * Np));
+    OJMI_HIP(hipMemcpy(f.data(), c->dF, f.size() * 4, hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < R; ++r)
+      for (int64_t j = 0; j < n; ++j) out[r * n + j] = (double)f[(size_t)(r * Np + j)];
+  }
   return OJMI_OK;
 }
 
 static int compute_energies(ojmi_ctx* c) {
+  if (c->energies_valid) return OJMI_OK;
   const int64_t C = c->R * c->M;
   if (!c->dJ64) return fail(OJMI_ENOTSUP, "energies need the FP64 coupling copy");
   int rc;
-  if (!c->dH && (rc = dalloc(&c->dH, (size_t)(C * c->Npad)))) return rc;
+  if ((rc = ensure_mfma_spins(c))) return rc;
+  if (!c->dpart && (rc = dalloc(&c->dpart, 2 * energy_part_count((int)C, (int)c->Npad))))
+    return rc;
   const float* F32 = (c->fields_valid && c->dF && c->M == 1) ? (const float*)c->dF : nullptr;
-  hipError_t e = launch_energy_f64(c->dJ64, c->dh64, c->dspins, (int)c->n, (int)c->Npad, (int)C,
-                                   c->dH, c->dE, F32, F32 ? c->ddrift : nullptr, c->st);
+  hipError_t e = launch_pack_spins(c->dspins, (int)c->n, (int)C, (int)c->Npad, c->dS8, nullptr,
+                                   c->st);
+  if (e == hipSuccess)
+    e = launch_energy_f64(c->dS8, c->dJ64, c->dh64, (int)C, (int)c->Npad, F32, c->dpart,
+                          c->dpart + energy_part_count((int)C, (int)c->Npad), c->dE,
+                          F32 ? c->ddrift : nullptr, c->st);
   if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("energy kernel: ") + hipGetErrorString(e));
   OJMI_HIP(hipStreamSynchronize(c->st));
+  c->energies_valid = true;
   return OJMI_OK;
 }
 
@@ -1194,7 +1390,7 @@ int ojmi_energies(ojmi_ctx* c, double* out) {
 int ojmi_field_drift(ojmi_ctx* c, double* out) {
   if (!c || !out) return fail(OJMI_EINVAL, "null argument");
   *out = 0.0;
-  if (!(c->fields_valid && c->dF && c->M == 1)) return OJMI_OK;
+  if (!(c->fields_valid && c->dF && c->M == 1 && c->kind == 0)) return OJMI_OK;
   int rc = compute_energies(c);
   if (rc) return rc;
   std::vector<double> d((size_t)c->R);
@@ -1250,6 +1446,9 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->waves_per_replica =
       (c->family == OJMI_KERNEL_CSR_LEVEL) ? csr_waves_per_replica(c->R, c->D, c->csr_waves, (int)c->n, c->nlev, c->ftype) : 1;
   info->reserved = 0;
+  info->last_refreshes = c->last_refreshes;
+  info->last_flips = c->last_flips;
+  info->last_rows = c->last_rows;
   return OJMI_OK;
 }
 

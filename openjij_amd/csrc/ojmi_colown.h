@@ -265,6 +265,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
   constexpr int NQ = (NS + 3) / 4;
   const int nblocks = gridDim.x;
   uint32_t done = 0;
+  uint64_t nflip = 0, nrow = 0;  // workgroup-uniform counters
   typename TileVec<JT>::type tile[16 / G];
 
   if (a.T > 0) {
@@ -349,7 +350,9 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
           fl[g] = uniform64(sFlip[g]);
           sg[g] = uniform64(sSign[g]);
           U |= fl[g];
+          nflip += (uint64_t)__builtin_popcountll(fl[g]);
         }
+        nrow += (uint64_t)__builtin_popcountll(U);
         if (w == ow && holds) {
           constexpr uint64_t m = (VEC == 64) ? ~0ull : ((1ull << VEC) - 1);
 #pragma unroll
@@ -437,6 +440,10 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     }
   }
 
+  if (a.stats && tid == 0) {
+    atomicAdd(a.stats, (unsigned long long)nflip);
+    atomicAdd(a.stats + 1, (unsigned long long)nrow);
+  }
 #pragma unroll
   for (int g = 0; g < G; ++g) {
     if (rbase + g >= a.R) continue;

@@ -26,6 +26,7 @@ struct DenseArgs {
   const void* F_in;    // optional [R][Npad] initial fields (field type)
   void* F_out;         // optional [R][Npad] final fields
   int rule;            // classical acceptance rule: RULE_METROPOLIS / RULE_HEAT_BATH
+  unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
 };
 
 // Batch of independent dense classical problems (k_batch.hip, k_batch_wave):
@@ -45,6 +46,7 @@ struct BatchArgs {
   const double* p0_d;    // [K][T] -1/beta per sweep
   const float* p0_f;
   int rule;
+  unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
 };
 
 // CSR in level order (k_csr.hip): spins sorted by level, neighbours in ELL
@@ -65,6 +67,7 @@ struct CsrArgs {
   const double* p0_d;
   const float* p0_f;
   int rule;
+  unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
 };
 
 // Workgroup-shared dense sweep (k_shared.hip): J in natural row layout
@@ -86,6 +89,7 @@ struct SharedArgs {
   int xcd_spin;         // bound on pacing polls per block
   uint32_t* progress;   // [workgroups] blocks completed
   int rule;
+  unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
 };
 
 size_t shared_lds_bytes(int NS, int G, int ring_rows, int fbytes);
@@ -120,10 +124,23 @@ int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int 
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, int waves,
                             hipStream_t st);
 
-// FP64 fields/energies: H[c][j] = sum_i S[c][i] J64[i][j] (c over C configurations),
-// E[c] = sum_j s_j (H_j/2 + h_j); optional drift = max |F32[c][j] - (H + h)|.
-hipError_t launch_energy_f64(const double* J64, const double* h64, const int8_t* spins, int n,
-                             int Npad, int C, double* H, double* E, const float* F32,
-                             double* drift, hipStream_t st);
+// Matrix-core contractions (k_spin_gemm.hip).  S8 / Sbf: spins zero-padded to
+// [C][Npad] (int8 / bf16 bits), written by launch_pack_spins; J matrices are
+// the natural zero-padded Npad x Npad couplings (Npad a multiple of 64).
+hipError_t launch_pack_spins(const int8_t* spins, int n, int C, int Kp, int8_t* S8,
+                             uint16_t* Sbf, hipStream_t st);
+// F[c][j] = h[j] + sum_i S[c][i] (Jhi + Jlo)[i][j], FP32 accumulation (bf16 MFMA)
+hipError_t launch_fields_bf16x2(const uint16_t* Sbf, const uint16_t* Jhi, const uint16_t* Jlo,
+                                const float* h, int C, int Npad, float* F, hipStream_t st);
+// F[c][j] = h[j] + sum_i S[c][i] J8[i][j], exact int32 (int8 MFMA)
+hipError_t launch_fields_i8(const int8_t* S8, const int8_t* J8, const int32_t* h, int C, int Npad,
+                            int32_t* F, hipStream_t st);
+// FP64 MFMA: E[c] = sum_j s_cj (H_cj / 2 + h_j) with H = S J64, and, when F32 is
+// given, drift[c] = max_j |F32[c][j] - (H_cj + h_j)|.  part_e / part_d hold
+// energy_part_count(C, Npad) doubles each.
+size_t energy_part_count(int C, int Npad);
+hipError_t launch_energy_f64(const int8_t* S8, const double* J64, const double* h64, int C,
+                             int Npad, const float* F32, double* part_e, double* part_d,
+                             double* E, double* drift, hipStream_t st);
 
 }  // namespace ojmi

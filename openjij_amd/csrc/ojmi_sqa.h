@@ -64,6 +64,7 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
   constexpr int NQ = (NS + 3) / 4;
   const int upper = (M & 1) ? M - 1 : M;
 
+  uint64_t nflip = 0;  // wave-uniform: one row per flip (one wave per slice)
   for (int tt = 0; tt < a.T; ++tt) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)tt);
     const FT at = sweep_param<TT>(a, tt);
@@ -114,6 +115,7 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
                 sb ^= 1ull << i0;
                 if (lane == i0) dw ^= 1ull << k;
                 row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
+                ++nflip;
               }
               if (lane == 0) s_down[t * NS + k] = sb;
             }
@@ -124,6 +126,10 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
     }
   }
 
+  if (a.stats && lane == 0) {
+    atomicAdd(a.stats, (unsigned long long)nflip);
+    atomicAdd(a.stats + 1, (unsigned long long)nflip);
+  }
 #pragma unroll
   for (int pp = 0; pp < PPW; ++pp) {
 #pragma unroll

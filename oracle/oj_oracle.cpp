@@ -568,7 +568,8 @@ int orc_brute_force_dense(i64 n, const double* J, double* emin, i8* best) {
 }  // extern "C"
 template <typename F>
 static void replay_dense_T(int mode, i64 n, const double* J, i64 R, i8* spins, u64 seed,
-                           u64 t0, const double* nbinv, i64 T, u32 word3, int rule) {
+                           u64 t0, const double* nbinv, i64 T, u32 word3, int rule,
+                           const double* F_init = nullptr) {
   const i64 n1 = n + 1;
   std::vector<F> Jc((size_t)n * n), h(n);
   for (i64 i = 0; i < n; ++i) {
@@ -581,12 +582,17 @@ static void replay_dense_T(int mode, i64 n, const double* J, i64 R, i8* spins, u
   for (i64 r = 0; r < R; ++r) {
     i8* s = spins + r * n;
     std::vector<F> Fv(h);
-    for (i64 k = 0; k < n; ++k) {
-      const F* row = &Jc[k * n];
-      if (s[k] > 0)
-        for (i64 j = 0; j < n; ++j) Fv[j] = Fv[j] + row[j];
-      else
-        for (i64 j = 0; j < n; ++j) Fv[j] = Fv[j] - row[j];
+    if (F_init) {
+      // the fields the run starts from, given exactly in the field type
+      for (i64 j = 0; j < n; ++j) Fv[j] = (F)F_init[r * n + j];
+    } else {
+      for (i64 k = 0; k < n; ++k) {
+        const F* row = &Jc[k * n];
+        if (s[k] > 0)
+          for (i64 j = 0; j < n; ++j) Fv[j] = Fv[j] + row[j];
+        else
+          for (i64 j = 0; j < n; ++j) Fv[j] = Fv[j] - row[j];
+      }
     }
     for (i64 t = 0; t < T; ++t) {
       const u64 tg = t0 + t;
@@ -623,6 +629,26 @@ int orc_replay_dense_wr(int mode, i64 n, const double* J_ext, i64 R, i8* spins, 
   else if (mode == 2) {
     if (!is_integer_matrix(J_ext, (n + 1) * (n + 1))) return 2;
     replay_dense_T<int32_t>(2, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, rule);
+  } else return 1;
+  return 0;
+}
+
+// As orc_replay_dense_wr, starting every replica r from the fields
+// F_init[r][0..n-1] instead of accumulating them from the spins (the engine's
+// matrix-core refresh points).  FP32 mode rounds them to float, which is exact
+// for fields downloaded from an FP32 context.
+int orc_replay_dense_from_fields(int mode, i64 n, const double* J_ext, i64 R, i8* spins,
+                                 const double* F_init, u64 seed, u64 sweep_offset,
+                                 const double* nbinv, i64 T, u32 word3, int rule) {
+  if (n < 1 || R < 0 || T < 0 || rule < 0 || rule > 1 || !F_init) return 1;
+  if (mode == 0)
+    replay_dense_T<double>(0, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, rule, F_init);
+  else if (mode == 1)
+    replay_dense_T<float>(1, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, rule, F_init);
+  else if (mode == 2) {
+    if (!is_integer_matrix(J_ext, (n + 1) * (n + 1))) return 2;
+    replay_dense_T<int32_t>(2, n, J_ext, R, spins, seed, sweep_offset, nbinv, T, word3, rule,
+                            F_init);
   } else return 1;
   return 0;
 }

@@ -54,6 +54,8 @@ def lib():
             "orc_replay_dense_w": (C.c_int, [C.c_int, i64, vp, i64, vp, u64, u64, vp, i64, u32]),
             "orc_replay_dense_wr": (C.c_int, [C.c_int, i64, vp, i64, vp, u64, u64, vp, i64, u32,
                                               C.c_int]),
+            "orc_replay_dense_from_fields": (C.c_int, [C.c_int, i64, vp, i64, vp, vp, u64, u64,
+                                                       vp, i64, u32, C.c_int]),
             "orc_replay_csr": (C.c_int, [C.c_int, i64, vp, vp, vp, i64, vp, u64, u64, vp, i64]),
             "orc_replay_csr_r": (C.c_int, [C.c_int, i64, vp, vp, vp, i64, vp, u64, u64, vp, i64,
                                            C.c_int]),
@@ -222,6 +224,22 @@ def replay_dense(mode, J_ext, spins, seed, sweep_offset, nbinv, word3=0, rule=0)
     nb = _f64(nbinv)
     rc = lib().orc_replay_dense_wr(MODE[mode], J.shape[0] - 1, _p(J), s.shape[0], _p(s), seed,
                                    sweep_offset, _p(nb), len(nb), word3, rule)
+    assert rc == 0, rc
+    return s
+
+
+def replay_dense_from_fields(mode, J_ext, spins, fields, seed, sweep_offset, nbinv, word3=0,
+                             rule=0):
+    """replay_dense starting every replica from the given fields [R][n]
+    (instead of accumulating them from the spins)."""
+    J = _f64(J_ext)
+    s = _i8(spins).copy()
+    F = _f64(fields)
+    assert F.shape == s.shape
+    nb = _f64(nbinv)
+    rc = lib().orc_replay_dense_from_fields(MODE[mode], J.shape[0] - 1, _p(J), s.shape[0], _p(s),
+                                            _p(F), seed, sweep_offset, _p(nb), len(nb), word3,
+                                            rule)
     assert rc == 0, rc
     return s
 
