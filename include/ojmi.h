@@ -207,6 +207,22 @@ int ojmi_solution(ojmi_ctx* ctx, int8_t* out);
 int ojmi_energies(ojmi_ctx* ctx, double* out);
 
 /*
+ * ojmi_energies without allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * ojmi_energies, bit for bit (same kernels); the padded spins and per-tile
+ * partials it needs live in a static device array of 16 MiB per GPU
+ * (allocated with the library's code object, held until the process ends),
+ * so (R * M) * Npad + 16 * (R * M) * (Npad / 64) + 8 * (R * M) bytes must fit
+ * in 16 MiB (Npad = 64 * 2^ceil(log2(ceil(n / 64))) up to 4096 spins, else
+ * 64 * ceil(n / 64)); larger contexts return OJMI_ERANGE.  Every call on a
+ * GPU uses that GPU's one array: calls that overlap in time on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller.
+ * Batch contexts return OJMI_ENOTSUP.
+ */
+int ojmi_energies_static(ojmi_ctx* ctx, double* out);
+
+/*
  * Local fields F = h + J s of every replica from its current spins, on the
  * matrix cores, kept in the context (the fields the next sweep starts from
  * when refresh_every > 0).  Replaces ClassicalIsing::reset_dE,

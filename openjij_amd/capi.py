@@ -68,6 +68,7 @@ def lib():
             "ojmi_create_batch": (C.c_int, [i64, vp, vp, i64, i32, C.POINTER(vp)]),
             "ojmi_run_batch": (C.c_int, [vp, vp, vp, i64, u64, u64]),
             "ojmi_refresh_fields": (C.c_int, [vp]),
+            "ojmi_energies_static": (C.c_int, [vp, vp]),
             "ojmi_get_fields": (C.c_int, [vp, vp]),
         }
         for name, (res, args) in sig.items():

@@ -34,6 +34,7 @@
 #include "ojmi_kernels.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace ojmi {
 
@@ -311,6 +312,34 @@ __global__ __launch_bounds__(256) void k_energy_parts(const double* __restrict__
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Static scratch of the allocation-free energies (ojmi_energies_static): the
+// padded spins and the per-tile partials live in this file-scope device
+// array, one copy per GPU, held until the process ends.  In host code the
+// name `scratch` is only a stand-in; kernels get the device address looked up
+// by hipGetSymbolAddress on each device.
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+hipError_t static_scratch(int device, void** ptr) {
+  static std::mutex mu;
+  static void* addr[MAX_DEVICES] = {};
+  if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[device]) {
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
+}
 
 hipError_t launch_pack_spins(const int8_t* spins, int n, int C, int Kp, int8_t* S8,
                              uint16_t* Sbf, hipStream_t st) {

@@ -124,6 +124,7 @@ struct ojmi_ctx {
 
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  int device = 0;  // current device when the context was created
 
   int64_t last_T = 0, last_attempts = 0;
   double last_ms = 0, last_refresh_ms = 0;
@@ -210,6 +211,7 @@ int device_lds_budget() {
 }
 
 int common_init(ojmi_ctx* c) {
+  OJMI_HIP(hipGetDevice(&c->device));
   c->lds_budget = device_lds_budget();
   OJMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   OJMI_HIP(hipEventCreate(&c->e0));
@@ -1384,6 +1386,37 @@ int ojmi_energies(ojmi_ctx* c, double* out) {
   int rc = compute_energies(c);
   if (rc) return rc;
   OJMI_HIP(hipMemcpy(out, c->dE, (size_t)(c->R * c->M) * 8, hipMemcpyDeviceToHost));
+  return OJMI_OK;
+}
+
+int ojmi_energies_static(ojmi_ctx* c, double* out) {
+  if (!c || !out) return fail(OJMI_EINVAL, "null argument");
+  if (c->kind == 2) return fail(OJMI_ENOTSUP, "ojmi_energies_static: batch contexts use ojmi_energies");
+  if (!c->dJ64) return fail(OJMI_ENOTSUP, "energies need the FP64 coupling copy");
+  const int64_t C = c->R * c->M, Np = c->Npad;
+  const size_t parts = energy_part_count((int)C, (int)Np);
+  const size_t s8 = (((size_t)(C * Np) + 255) / 256) * 256;
+  const size_t need = s8 + 2 * parts * 8 + (size_t)C * 8;
+  if (need > STATIC_SCRATCH_BYTES)
+    return fail(OJMI_ERANGE, "ojmi_energies_static: " + std::to_string(C) + " configurations x " +
+                                 std::to_string(Np) + " padded spins need " +
+                                 std::to_string(need) + " bytes of scratch; the static buffer holds " +
+                                 std::to_string(STATIC_SCRATCH_BYTES));
+  void* base = nullptr;
+  hipError_t e = static_scratch(c->device, &base);
+  if (e != hipSuccess)
+    return fail(OJMI_EDEVICE, std::string("static scratch lookup: ") + hipGetErrorString(e));
+  int8_t* S8 = static_cast<int8_t*>(base);
+  double* part_e = reinterpret_cast<double*>(static_cast<char*>(base) + s8);
+  double* part_d = part_e + parts;
+  double* E = part_d + parts;
+  e = launch_pack_spins(c->dspins, (int)c->n, (int)C, (int)Np, S8, nullptr, c->st);
+  if (e == hipSuccess)
+    e = launch_energy_f64(S8, c->dJ64, c->dh64, (int)C, (int)Np, nullptr, part_e, part_d, E,
+                          nullptr, c->st);
+  if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("energy kernel: ") + hipGetErrorString(e));
+  OJMI_HIP(hipMemcpyAsync(out, E, (size_t)C * 8, hipMemcpyDeviceToHost, c->st));
+  OJMI_HIP(hipStreamSynchronize(c->st));
   return OJMI_OK;
 }
 

@@ -139,6 +139,12 @@ hipError_t launch_fields_i8(const int8_t* S8, const int8_t* J8, const int32_t* h
 // given, drift[c] = max_j |F32[c][j] - (H_cj + h_j)|.  part_e / part_d hold
 // energy_part_count(C, Npad) doubles each.
 size_t energy_part_count(int C, int Npad);
+// Device address (on `device`) of the static scratch array of the
+// allocation-free energies, STATIC_SCRATCH_BYTES long; looked up once per
+// device with that device current.
+constexpr size_t STATIC_SCRATCH_BYTES = (size_t)16 << 20;
+constexpr int MAX_DEVICES = 64;
+hipError_t static_scratch(int device, void** ptr);
 hipError_t launch_energy_f64(const int8_t* S8, const double* J64, const double* h64, int C,
                              int Npad, const float* F32, double* part_e, double* part_d,
                              double* E, double* drift, hipStream_t st);
