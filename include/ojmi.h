@@ -141,6 +141,23 @@ int ojmi_run_batch(ojmi_ctx* ctx, const double* beta, const int64_t* one_mc_step
                    uint64_t seed, uint64_t sweep_offset);
 
 /*
+ * Statistics of the automatic beta bounds for every problem of a batch
+ * context, computed on the GPU from the batch's couplings (the dE analysis of
+ * geometric_ising_beta_schedule, openjij/sampler/sa_sampler.py:882-911,937-938).
+ * spins: [K][2][n_k] int8 +-1, the two random spin vectors of each problem
+ * (the reference's random_spin[:-1, c] for c = 0, 1).  out: [K][5] doubles
+ * {dE_max, dE_min, lin_max, mean_abs, int_min}: with dE_quad = -2 s (J s) and
+ * threshold 1e-8, dE_max / dE_min are the median / minimum of the entries
+ * above the threshold (1.1 / 1 if none); over the entries with |dE_quad|
+ * above it, lin_max = max |h s| and mean_abs = mean |dE_quad| (-1 if none),
+ * so rate_dE = lin_max / (mean_abs + 1e-8) (0.5 if none); int_min is the
+ * smallest |J_ij| or |h_i| above the threshold (+inf if none; the reference
+ * also counts the dummy corner 1).  The bounds follow by the reference's
+ * scalar formulas (sa_sampler.py:914-968).
+ */
+int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
+
+/*
  * Tuning knobs (no reference counterpart).  Keys:
  *   "refresh_every": classical dense FP32 contexts: K > 0 computes the local
  *                    fields on the matrix cores (ojmi_refresh_fields) at the

@@ -69,6 +69,7 @@ def lib():
             "ojmi_run_batch": (C.c_int, [vp, vp, vp, i64, u64, u64]),
             "ojmi_refresh_fields": (C.c_int, [vp]),
             "ojmi_energies_static": (C.c_int, [vp, vp]),
+            "ojmi_batch_beta_stats": (C.c_int, [vp, vp, vp]),
             "ojmi_get_fields": (C.c_int, [vp, vp]),
         }
         for name, (res, args) in sig.items():
@@ -245,3 +246,14 @@ class BatchContext(Context):
         e = np.empty((len(self.ns), self.R), np.float64)
         _check(lib().ojmi_energies(self.h, _p(e)))
         return e
+
+    def beta_stats(self, spin_pairs):
+        """spin_pairs: list of [2][n_k] +-1 arrays.  Returns [K][5]: dE_max,
+        dE_min, lin_max, mean_abs, int_min of each problem (ojmi_batch_beta_stats)."""
+        flat = np.concatenate([np.ascontiguousarray(s, dtype=np.int8).ravel()
+                               for s in spin_pairs])
+        if flat.size != 2 * sum(self.ns):
+            raise ValueError("need two spin vectors of n_k entries per problem")
+        out = np.empty((len(self.ns), 5), np.float64)
+        _check(lib().ojmi_batch_beta_stats(self.h, _p(flat), _p(out)))
+        return out

@@ -1111,6 +1111,52 @@ int ojmi_run_batch(ojmi_ctx* c, const double* beta, const int64_t* mcs, int64_t 
   return run_batch(c, p0, T, seed, sweep_offset);
 }
 
+int ojmi_batch_beta_stats(ojmi_ctx* c, const int8_t* spins, double* out) {
+  if (!c || !spins || !out) return fail(OJMI_EINVAL, "null argument");
+  if (c->kind != 2) return fail(OJMI_EINVAL, "ojmi_batch_beta_stats needs a batch context");
+  const int64_t K = c->K;
+  std::vector<int64_t> off((size_t)K);
+  int64_t tot = 0;
+  int nmax = 1;
+  for (int64_t k = 0; k < K; ++k) {
+    off[k] = tot;
+    tot += 2 * (int64_t)c->bn[k];
+    nmax = std::max(nmax, (int)c->bn[k]);
+  }
+  for (int64_t q = 0; q < tot; ++q)
+    if (spins[q] != 1 && spins[q] != -1) return fail(OJMI_EINVAL, "spins must be +1 or -1");
+  if (beta_stats_lds_bytes(nmax) > (size_t)c->lds_budget)
+    return fail(OJMI_ENOTSUP, "beta statistics: problem too large for LDS");
+  int8_t* dS = nullptr;
+  int64_t* doff = nullptr;
+  double* dout = nullptr;
+  int rc;
+  if ((rc = dalloc(&dS, (size_t)tot)) || (rc = dalloc(&doff, (size_t)K)) ||
+      (rc = dalloc(&dout, (size_t)(5 * K)))) {
+    if (dS) (void)hipFree(dS);
+    if (doff) (void)hipFree(doff);
+    return rc;
+  }
+  BatchArgs b{};
+  b.J = c->dJ;
+  b.h = c->dh;
+  b.j_off = c->d_bj_off;
+  b.h_off = c->d_bh_off;
+  b.n = c->d_bn;
+  hipError_t e = hipMemcpy(dS, spins, (size_t)tot, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(doff, off.data(), (size_t)K * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_batch_beta_stats(c->ftype, c->jtype, b, c->d_brb, (int)K, nmax, dS, doff, dout,
+                                c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)(5 * K) * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(dS);
+  (void)hipFree(doff);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("beta statistics: ") + hipGetErrorString(e));
+  return OJMI_OK;
+}
+
 int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   if (!c || !key) return fail(OJMI_EINVAL, "null argument");
   const std::string k(key);

@@ -124,6 +124,14 @@ int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int 
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, int waves,
                             hipStream_t st);
 
+// Automatic-schedule statistics of every problem of a batch (k_beta_stats.hip):
+// out[k] = {dE_max, dE_min, lin_max, mean_abs, int_min} from the spin pair
+// S2[s2_off[k] .. + 2 n_k) (two rows of n_k +-1 values).
+size_t beta_stats_lds_bytes(int nmax);
+hipError_t launch_batch_beta_stats(int ftype, int jtype, const BatchArgs& b, const int64_t* rb,
+                                   int K, int nmax, const int8_t* S2, const int64_t* s2_off,
+                                   double* out, hipStream_t st);
+
 // Matrix-core contractions (k_spin_gemm.hip).  S8 / Sbf: spins zero-padded to
 // [C][Npad] (int8 / bf16 bits), written by launch_pack_spins; J matrices are
 // the natural zero-padded Npad x Npad couplings (Npad a multiple of 64).

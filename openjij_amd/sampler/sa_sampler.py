@@ -39,41 +39,47 @@ def _precision_code(p) -> int:
     return _PRECISIONS[key]
 
 
-def geometric_ising_beta_schedule(cxxgraph, beta_max=None, beta_min=None, num_sweeps=1000,
-                                  seed=None):
-    """Geometric beta schedule with automatic bounds, as the reference builds it
-    (openjij/sampler/sa_sampler.py:853-976): the bounds come from local-field
-    statistics of two random spin vectors drawn with NumPy's legacy global
-    stream after ``np.random.seed(seed)``; one_mc_step = max(1, num_sweeps // 1000).
-    Returns (schedule list, [beta_max, beta_min])."""
-    if seed is not None:
-        np.random.seed(seed)
-    tiny = 1e-8
-    dE_max, dE_min, rate = 1.0, 1.0, 0.5
-    mat = cxxgraph.get_interactions()
+_TINY = 1e-8
+
+
+def beta_stats_host(mat, spins):
+    """dE statistics of the automatic bounds (openjij/sampler/sa_sampler.py:882-911)
+    for the (n+1)^2 interaction matrix `mat` (dense or scipy) and a random spin
+    matrix `spins` of shape (n+1, 2) whose last row is 1.  Returns
+    (dE_max, dE_min, rate_dE)."""
     dense = isinstance(mat, np.ndarray)
-    n1 = mat.shape[0]
-    if beta_min is None or beta_max is None:
-        if n1 <= 1:
-            dE_max = dE_min = 1
-        else:
-            spins = np.random.choice([-1, 1], size=(n1, 2))
-            spins[-1, :] = 1
-            quad = mat[:-1, :-1] @ spins[:-1, :] * (-2 * spins[:-1, :])
-            lin_col = mat[:-1, -1][:, np.newaxis] if dense else mat[:-1, -1].toarray()
-            lin = lin_col * spins[:-1]
-            quad_abs = np.abs(quad)
-            big = quad_abs > tiny
-            if np.any(big):
-                rate = np.max(np.abs(lin[big]) / (quad_abs[big].mean() + tiny))
-            pos = quad[quad > tiny]
-            if len(pos) == 0:
-                dE_min, dE_max = 1, 1.1
-            else:
-                dE_max = np.median(pos, axis=0).mean()
-                dE_min = np.min(pos, axis=0).mean()
+    dE_max, dE_min, rate = 1.0, 1.0, 0.5
+    quad = mat[:-1, :-1] @ spins[:-1, :] * (-2 * spins[:-1, :])
+    lin_col = mat[:-1, -1][:, np.newaxis] if dense else mat[:-1, -1].toarray()
+    lin = lin_col * spins[:-1]
+    quad_abs = np.abs(quad)
+    big = quad_abs > _TINY
+    if np.any(big):
+        rate = np.max(np.abs(lin[big]) / (quad_abs[big].mean() + _TINY))
+    pos = quad[quad > _TINY]
+    if len(pos) == 0:
+        dE_min, dE_max = 1, 1.1
+    else:
+        dE_max = np.median(pos, axis=0).mean()
+        dE_min = np.min(pos, axis=0).mean()
+    return dE_max, dE_min, rate
+
+
+def int_min_host(mat):
+    """Smallest |entry| above the threshold of the (n+1)^2 layout (sa_sampler.py:937-938)."""
+    absm = np.abs(mat)
+    if not isinstance(mat, np.ndarray):
+        absm = absm.toarray()
+    return np.min(absm[absm > _TINY])
+
+
+def beta_bounds(dE_max, dE_min, rate, int_min, n1, num_sweeps, beta_min=None, beta_max=None):
+    """The scalar formulas of the automatic bounds (sa_sampler.py:914-968):
+    beta_min by a five-step bisection of 2 beta p_init = exp(-beta dE_min) -
+    exp(-beta dE_max), beta_max from int_min / dE_min; `int_min` is a value or a
+    callable (evaluated only when beta_max is None).  Returns (beta_min, beta_max)."""
     n = n1
-    p_init = 9 / 10 - 8 / 10 * np.tanh(rate / 50) + tiny
+    p_init = 9 / 10 - 8 / 10 * np.tanh(rate / 50) + _TINY
     p_final = 1 / 1000
     if beta_min is None:
         lo, hi = -np.log(p_init) / dE_max, -np.log(p_init) / dE_min
@@ -90,29 +96,71 @@ def geometric_ising_beta_schedule(cxxgraph, beta_max=None, beta_min=None, num_sw
         bmm = beta_min * 0.8
         beta_min = bmm + (beta_min * 1.5 - bmm) * np.tanh(num_sweeps / n)
     if beta_max is None:
-        absm = np.abs(mat)
-        if not dense:
-            absm = absm.toarray()
-        int_min = np.min(absm[absm > tiny])
-        int_min = max(int_min, dE_min / 10.0)
-        b_int = -np.log(p_final) / int_min
+        im = int_min() if callable(int_min) else int_min
+        im = max(im, dE_min / 10.0)
+        b_int = -np.log(p_final) / im
         b_dE = -np.log(p_final) / dE_min
         if rate < 2:
             lo_, hi_ = min(b_dE, b_int), max(b_dE, b_int)
             beta_max = lo_ + (hi_ - lo_) * np.tanh(num_sweeps / (2 * n))
         else:
             last = min(100, num_sweeps // 10 + 1)
-            beta_max = -np.log(p_final) / max(dE_min, int_min) / last
+            beta_max = -np.log(p_final) / max(dE_min, im) / last
         if beta_max < beta_min:
             beta_max = beta_min * 10.0
     if rate > 1:
         beta_min *= rate
         beta_max *= rate
+    return beta_min, beta_max
+
+
+def device_beta_stats(row, scale=1.0):
+    """(dE_max, dE_min, rate_dE, int_min) of the reference's analysis from one
+    row of ojmi_batch_beta_stats computed on couplings scaled by `scale`
+    (1 or 4, so the divisions are exact)."""
+    dE_max, dE_min, lin_max, mean_abs, im = (float(x) for x in row)
+    dE_max, dE_min = dE_max / scale, dE_min / scale
+    rate = 0.5 if mean_abs < 0 else (lin_max / scale) / (mean_abs / scale + _TINY)
+    return dE_max, dE_min, rate, min(im / scale, 1.0)
+
+
+def random_spin_pair(n1, seed=None):
+    """The reference's random spin matrix: np.random.choice([-1, 1], size=(n1, 2))
+    after np.random.seed(seed), last row set to 1 (sa_sampler.py:876-890)."""
+    if seed is not None:
+        np.random.seed(seed)
+    spins = np.random.choice([-1, 1], size=(n1, 2))
+    spins[-1, :] = 1
+    return spins
+
+
+def geometric_schedule(beta_min, beta_max, num_sweeps):
     per_beta = max(1, num_sweeps // 1000)
-    schedule = cxxjij.utility.make_classical_schedule_list(
+    return cxxjij.utility.make_classical_schedule_list(
         beta_min=beta_min, beta_max=beta_max, one_mc_step=per_beta,
         num_call_updater=num_sweeps // per_beta)
-    return schedule, [beta_max, beta_min]
+
+
+def geometric_ising_beta_schedule(cxxgraph, beta_max=None, beta_min=None, num_sweeps=1000,
+                                  seed=None):
+    """Geometric beta schedule with automatic bounds, as the reference builds it
+    (openjij/sampler/sa_sampler.py:853-976): the bounds come from local-field
+    statistics of two random spin vectors drawn with NumPy's legacy global
+    stream after ``np.random.seed(seed)``; one_mc_step = max(1, num_sweeps // 1000).
+    Returns (schedule list, [beta_max, beta_min])."""
+    if seed is not None:
+        np.random.seed(seed)
+    mat = cxxgraph.get_interactions()
+    n1 = mat.shape[0]
+    dE_max, dE_min, rate = 1.0, 1.0, 0.5
+    if beta_min is None or beta_max is None:
+        if n1 <= 1:
+            dE_max = dE_min = 1
+        else:
+            dE_max, dE_min, rate = beta_stats_host(mat, random_spin_pair(n1))
+    beta_min, beta_max = beta_bounds(dE_max, dE_min, rate, lambda: int_min_host(mat), n1,
+                                     num_sweeps, beta_min, beta_max)
+    return geometric_schedule(beta_min, beta_max, num_sweeps), [beta_max, beta_min]
 
 
 def schedule_arrays(schedule):
@@ -276,17 +324,12 @@ class SASampler(BaseSampler):
         sweeps = int(self._params["num_sweeps"])
         seed_v = resolve_seed(seed)
         models = list(models)
-        idx, J_exts, betas, scales, mcs = [], [], [], [], None
+        idx, J_exts, scales = [], [], []
         for k, bqm in enumerate(models):
             if bqm.num_variables == 0:
                 continue
             dense = BinaryQuadraticModel(bqm.linear, bqm.quadratic, bqm.offset, bqm.vartype,
                                          sparse=False)
-            graph, _ = dense.get_cxxjij_ising_graph()
-            sched, _ = geometric_ising_beta_schedule(
-                graph, beta_max=self._params["beta_max"], beta_min=self._params["beta_min"],
-                num_sweeps=sweeps, seed=seed)
-            beta, m = schedule_arrays(sched)
             h, J, _ = dense.ising_arrays()
             n = dense.num_variables
             ext = np.zeros((n + 1, n + 1))
@@ -299,15 +342,36 @@ class SASampler(BaseSampler):
             ext[n, n] = 1.0
             idx.append(k)
             J_exts.append(ext)
-            betas.append(beta / scale)
             scales.append(scale)
-            mcs = m
         t0 = time.perf_counter()
         spins = {}
         exec_us = 0.0
+        self.schedule_info = {}
         if idx:
             ctx = capi.BatchContext.create(J_exts, R, _precision_code(precision))
             try:
+                bmin, bmax = self._params["beta_min"], self._params["beta_max"]
+                stats = None
+                if bmin is None or bmax is None:
+                    # the reference's random spin pair of each model (the same
+                    # draws as one sample() call per model with this seed), and
+                    # the dE statistics of all models at once on the GPU
+                    pairs = [random_spin_pair(E.shape[0], seed)[:-1].T for E in J_exts]
+                    stats = ctx.beta_stats(pairs)
+                betas, bounds, mcs = [], [], None
+                for i, E in enumerate(J_exts):
+                    if stats is None:
+                        lo, hi = bmin, bmax
+                    else:
+                        lo, hi = beta_bounds(*device_beta_stats(stats[i], scales[i]),
+                                             E.shape[0], sweeps, bmin, bmax)
+                    beta, m = schedule_arrays(geometric_schedule(lo, hi, sweeps))
+                    betas.append(beta / scales[i])
+                    bounds.append((float(lo), float(hi)))
+                    mcs = m
+                self.schedule_info = {"beta_min": [b[0] for b in bounds],
+                                      "beta_max": [b[1] for b in bounds],
+                                      "num_sweeps": sweeps, "auto_bounds_on_gpu": stats is not None}
                 init = [initial_states(R, E.shape[0] - 1, seed_v, 0x5C + i)
                         for i, E in enumerate(J_exts)]
                 ctx.set_spins(init)
