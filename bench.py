@@ -324,7 +324,8 @@ def profiled(kernel_name, workload_name):
 
 
 KERNEL_NAMES = {1: "k_dense_wave", 2: "k_sqa_wave", 3: "k_csr_level", 4: "k_dense_shared",
-                5: "k_dense_colown", 6: "k_batch_wave", 7: "k_dense_lds"}
+                5: "k_dense_colown", 6: "k_batch_wave", 7: "k_dense_lds", 8: "k_batch_persist",
+                9: "k_batch_wave"}
 
 
 def kernel_name(info):

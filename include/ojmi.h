@@ -57,6 +57,8 @@ extern "C" {
 #define OJMI_KERNEL_DENSE_COLOWN 5 /* as 4, each wave owns a slice of the fields */
 #define OJMI_KERNEL_BATCH_WAVE 6   /* batch of problems, one wave per replica   */
 #define OJMI_KERNEL_DENSE_LDS 7    /* N > 4096: one workgroup per replica, fields in LDS */
+#define OJMI_KERNEL_BATCH_PERSIST 8 /* batch of problems, one persistent launch, LPT queues */
+#define OJMI_KERNEL_BATCH_FUSED 9   /* batch of problems, one launch, largest problem first */
 
 typedef struct ojmi_ctx ojmi_ctx;
 
@@ -119,7 +121,9 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
 /*
  * Batch context: num_problems independent classical dense problems of
  * different sizes n[k] (1..4096), each annealed as num_replicas replicas in
- * one launch per slot class.  No single reference function corresponds: it
+ * one launch whose workgroups hold the problems largest first, one wave per
+ * (problem, replica) (option "batch_mode": 1 runs one launch per slot class,
+ * 2 one persistent launch over per-XCD work queues; results are identical).  No single reference function corresponds: it
  * replaces running SASampler.sample / Algorithm<SingleSpinFlip>::run once per
  * problem and read (openjij/sampler/sampler.py:115-140).  J_ext holds the
  * problems' (n[k]+1)^2 matrices back to back, each in the ojmi_create_dense
@@ -174,6 +178,11 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *   "round_rows":    shared kernel: rows per staging round (0 = auto)
  *   "colown":        1 (default) = run shared contexts with the column-owned
  *                    kernel where the shape allows it, 0 = LDS row staging
+ *   "batch_mode":    batch contexts: 0 (default) one launch, problems largest
+ *                    first; 1 one launch per slot class; 2 one persistent
+ *                    launch pulling (problem, replica) items from per-XCD
+ *                    longest-first queues; results are identical
+ *   "batch_waves":   batch contexts, batch_mode 2: waves per workgroup (1..4)
  *   "csr_waves":     CSR contexts of degree <= 16: waves sharing one replica's
  *                    sweep (1, 2, 4, 8; 0 = chosen by the replica count)
  *   "update_method": acceptance rule of classical contexts (dense, CSR,

@@ -20,6 +20,9 @@ __device__ __forceinline__ int xcd_chunked(int b, int nb, int chunk) {
   return (idx / chunk) * round + xcd * chunk + (idx % chunk);
 }
 
+// NS is the slot capacity: a problem of n spins runs with its own slot count
+// ns = ceil(n / 64), the layout of its rows, so one launch can hold every
+// problem of a batch, largest first.
 template <typename FT, typename JT, int NS>
 __global__ __launch_bounds__(256) void k_batch_wave(BatchArgs b) {
   typedef typename ThreshT<FT>::type TT;
@@ -34,8 +37,9 @@ __global__ __launch_bounds__(256) void k_batch_wave(BatchArgs b) {
   const FT* h = reinterpret_cast<const FT*>(b.h) + b.h_off[k];
   int8_t* sp = b.spins + b.s_off[k] + (int64_t)r * n;
   const TT* p0 = p0_ptr<TT>(b.p0_d, b.p0_f) + (int64_t)k * b.T;
+  const int ns = (n + 63) >> 6;
   anneal_replica<FT, JT, NS>(J, h, sp, n, nullptr, nullptr, (uint32_t)r, (uint32_t)k, b.seed,
-                             b.t0, b.T, p0, b.rule, lane, b.stats);
+                             b.t0, b.T, p0, b.rule, lane, b.stats, ns);
 }
 
 template <typename FT, typename JT>

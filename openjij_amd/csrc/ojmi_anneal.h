@@ -73,9 +73,10 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
                                                uint32_t slice, uint64_t seed, uint64_t t0,
                                                int T, const typename ThreshT<FT>::type* p0,
                                                int rule, int lane,
-                                               unsigned long long* stats) {
+                                               unsigned long long* stats, int ns = NS) {
   typedef typename ThreshT<FT>::type TT;
-  constexpr int64_t RB = row_bytes<JT, NS>();
+  // ns (wave-uniform, <= NS): the problem's slot count; NS is the capacity
+  const int64_t RB = (ns == NS) ? row_bytes<JT, NS>() : row_bytes_rt<JT>(ns);
   uint64_t down = 0;
 #pragma unroll
   for (int m = 0; m < NS; ++m) {
@@ -87,11 +88,11 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
 #pragma unroll
     for (int m = 0; m < NS; ++m) F.set(m, F_in[64 * m + lane]);
   } else {
-    init_fields_seq<JT, FT, NS>(F, J, h, down, n, lane);
+    init_fields_seq<JT, FT, NS>(F, J, h, down, n, lane, ns);
   }
 
   const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
-  constexpr int NQ = (NS + 3) / 4;
+  const int NQ = (ns + 3) / 4;
   uint64_t nflip = 0;  // wave-uniform: one coupling row per accepted flip
   for (int t = 0; t < T; ++t) {
     const uint32_t tg = (uint32_t)(t0 + (uint64_t)t);
@@ -102,7 +103,7 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
 #pragma unroll
       for (int jj = 0; jj < 4; ++jj) {
         const int k = 4 * Q + jj;
-        if (k >= NS) break;
+        if (k >= ns) break;
         const int i = 64 * k + lane;
         const TT th = (i < n) ? accept_log<TT>(w[jj], rule) * nb : neg_inf<TT>();
         uint64_t live = ballot64(i < n);
@@ -120,7 +121,7 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
           live &= lanes_above(i0);
           const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
           flips |= 1ull << i0;
-          row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c);
+          row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c, ns);
         }
         down ^= ((flips >> lane) & 1ull) << k;
         nflip += (uint64_t)__builtin_popcountll(flips);

@@ -12,6 +12,7 @@
 
 #include "../../include/ojmi.h"
 #include "ojmi_kernels.h"
+#include "ojmi_persist.h"
 #include "ojmi_wave.h"
 
 using namespace ojmi;
@@ -136,6 +137,16 @@ struct ojmi_ctx {
   int64_t *d_bj_off = nullptr, *d_bh_off = nullptr, *d_bs_off = nullptr, *d_brb = nullptr;
   std::vector<std::pair<int, int32_t*>> bclass;  // (slot class NS, device problem list)
   std::vector<int> bclass_count;
+  // persistent batch launch: per-XCD LPT queues of (problem, replica) items
+  // 0: one launch of every problem, largest first (the hardware's in-order
+  // workgroup dispatch schedules them); 1: one launch per slot class;
+  // 2: one persistent launch over per-XCD software queues
+  int batch_mode = 0;
+  int32_t* d_fused = nullptr;  // every problem, largest first
+  int batch_waves = 4;  // waves per workgroup of the persistent launch
+  int batch_ns_max = 1;
+  int32_t *d_q_items = nullptr, *d_q_begin = nullptr;
+  uint32_t* d_q_head = nullptr;
   double* db_p0_d = nullptr;
   float* db_p0_f = nullptr;
   int64_t bTcap = 0;
@@ -146,7 +157,8 @@ struct ojmi_ctx {
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
                     dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
                     d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
-                    dJhi, dJlo, dS8, dSbf, dpart, dstats};
+                    dJhi, dJlo, dS8, dSbf, dpart, dstats, d_q_items, d_q_begin, d_q_head,
+                    d_fused};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     for (auto& cl : bclass)
@@ -460,14 +472,31 @@ static int run_batch(ojmi_ctx* c, const std::vector<double>& p0 /* [K][T] */, in
   b.rule = c->rule;
   if ((rc = begin_stats(c))) return rc;
   b.stats = c->dstats;
+  if (c->batch_mode == 2) OJMI_HIP(hipMemsetAsync(c->d_q_head, 0, 8 * sizeof(uint32_t), c->st));
   OJMI_HIP(hipEventRecord(c->e0, c->st));
-  // largest slot class first: the longest waves start first
-  for (size_t q = c->bclass.size(); q-- > 0;) {
-    b.inst = c->bclass[q].second;
-    b.num_items = c->bclass_count[q];
-    hipError_t e = launch_batch_wave(c->ftype, c->jtype, b, c->bclass[q].first, c->st);
+  if (c->batch_mode == 0) {
+    b.inst = c->d_fused;
+    b.num_items = (int)c->K;
+    hipError_t e = launch_batch_wave(c->ftype, c->jtype, b, c->batch_ns_max, c->st);
     if (e != hipSuccess)
       return fail(OJMI_EDEVICE, std::string("batch kernel launch: ") + hipGetErrorString(e));
+  } else if (c->batch_mode == 2) {
+    BatchQueues q{c->d_q_items, c->d_q_begin, c->d_q_head};
+    const int blocks = batch_persist_blocks(c->ftype, c->jtype, c->batch_ns_max, c->batch_waves);
+    if (blocks <= 0) return fail(OJMI_EDEVICE, "persistent batch kernel: no occupancy");
+    hipError_t e = launch_batch_persist(c->ftype, c->jtype, b, q, c->batch_ns_max, c->batch_waves,
+                                        blocks, c->st);
+    if (e != hipSuccess)
+      return fail(OJMI_EDEVICE, std::string("batch kernel launch: ") + hipGetErrorString(e));
+  } else {
+    // largest slot class first: the longest waves start first
+    for (size_t q = c->bclass.size(); q-- > 0;) {
+      b.inst = c->bclass[q].second;
+      b.num_items = c->bclass_count[q];
+      hipError_t e = launch_batch_wave(c->ftype, c->jtype, b, c->bclass[q].first, c->st);
+      if (e != hipSuccess)
+        return fail(OJMI_EDEVICE, std::string("batch kernel launch: ") + hipGetErrorString(e));
+    }
   }
   OJMI_HIP(hipEventRecord(c->e1, c->st));
   OJMI_HIP(hipEventSynchronize(c->e1));
@@ -1014,11 +1043,13 @@ int ojmi_create_batch(int64_t num_problems, const int64_t* n, const double* J_ex
   // layout: offsets, then one host image of couplings and fields
   const size_t fsz = (c->ftype == FT_F64) ? 8 : 4;
   std::vector<int64_t> j_off((size_t)K), h_off((size_t)K), s_off((size_t)K), rbv((size_t)K);
+  // rows of ceil(n_k / 64) slots (NSk); kernels of the power-of-two slot
+  // class pow2_slots(n_k) run them with that slot count as a bound
   std::vector<int> NSk((size_t)K);
   int64_t jbytes = 0, hcount = 0, scount = 0;
   c->bn.resize((size_t)K);
   for (int64_t k = 0; k < K; ++k) {
-    NSk[k] = pow2_slots(n[k]);
+    NSk[k] = (int)((n[k] + 63) / 64);
     rbv[k] = batch_row_bytes(c->jtype, NSk[k]);
     j_off[k] = jbytes;
     jbytes += ((n[k] * rbv[k] + 255) / 256) * 256;
@@ -1071,7 +1102,7 @@ int ojmi_create_batch(int64_t num_problems, const int64_t* n, const double* J_ex
   for (int NS = 1; NS <= 64; NS <<= 1) {
     std::vector<int32_t> lst;
     for (int64_t k = 0; k < K; ++k)
-      if (NSk[k] == NS) lst.push_back((int32_t)k);
+      if (pow2_slots(n[k]) == NS) lst.push_back((int32_t)k);
     if (lst.empty()) continue;
     std::stable_sort(lst.begin(), lst.end(), [&](int32_t a, int32_t b) { return n[a] > n[b]; });
     int32_t* d = nullptr;
@@ -1081,6 +1112,44 @@ int ojmi_create_batch(int64_t num_problems, const int64_t* n, const double* J_ex
     }
     c->bclass.push_back({NS, d});
     c->bclass_count.push_back((int)lst.size());
+    c->batch_ns_max = std::max(c->batch_ns_max, NS);
+  }
+  // longest first: cost of a replica ~ n_k attempts per sweep x 64 NS_k
+  // columns per row update
+  std::vector<int32_t> byc((size_t)K);
+  for (int64_t k = 0; k < K; ++k) byc[k] = (int32_t)k;
+  auto cost = [&](int32_t k) { return (double)n[k] * 64.0 * (double)NSk[k]; };
+  std::stable_sort(byc.begin(), byc.end(), [&](int32_t a, int32_t b) { return cost(a) > cost(b); });
+  if ((rc = up(&c->d_fused, byc))) {
+    delete c;
+    return rc;
+  }
+  c->family = OJMI_KERNEL_BATCH_FUSED;
+  // software queues: whole problems to 8 per-XCD queues in LPT order,
+  // replicas consecutive
+  if ((double)K * (double)num_replicas <= (double)INT32_MAX) {
+    std::vector<std::vector<int32_t>> qs(8);
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int32_t k : byc) {
+      int best = 0;
+      for (int q = 1; q < 8; ++q)
+        if (load[q] < load[best]) best = q;
+      load[best] += cost(k);
+      qs[best].push_back(k);
+    }
+    std::vector<int32_t> items, begin(9, 0);
+    items.reserve((size_t)(K * num_replicas));
+    for (int q = 0; q < 8; ++q) {
+      begin[q] = (int32_t)items.size();
+      for (int32_t k : qs[q])
+        for (int64_t r = 0; r < num_replicas; ++r) items.push_back((int32_t)(k * num_replicas + r));
+    }
+    begin[8] = (int32_t)items.size();
+    if ((rc = up(&c->d_q_items, items)) || (rc = up(&c->d_q_begin, begin)) ||
+        (rc = dalloc(&c->d_q_head, 8))) {
+      delete c;
+      return rc;
+    }
   }
   *out = c;
   return OJMI_OK;
@@ -1208,6 +1277,22 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     if (c->M != 1 && value != RULE_METROPOLIS)
       return fail(OJMI_EINVAL, "HEAT_BATH applies to classical systems (num_trotter = 1)");
     c->rule = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "batch_mode") {
+    if (c->kind != 2) return fail(OJMI_EINVAL, "batch_mode applies to batch contexts");
+    if (value < 0 || value > 2) return fail(OJMI_EINVAL, "batch_mode must be 0, 1 or 2");
+    if (value == 2 && !c->d_q_items)
+      return fail(OJMI_ENOTSUP, "persistent batch launch: too many (problem, replica) items");
+    c->batch_mode = (int)value;
+    c->family = value == 0 ? OJMI_KERNEL_BATCH_FUSED
+                           : (value == 1 ? OJMI_KERNEL_BATCH_WAVE : OJMI_KERNEL_BATCH_PERSIST);
+    return OJMI_OK;
+  }
+  if (k == "batch_waves") {
+    if (c->kind != 2) return fail(OJMI_EINVAL, "batch_waves applies to batch contexts");
+    if (value < 1 || value > 4) return fail(OJMI_EINVAL, "batch_waves must be 1..4");
+    c->batch_waves = (int)value;
     return OJMI_OK;
   }
   if (k == "csr_waves") {

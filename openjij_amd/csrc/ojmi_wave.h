@@ -68,18 +68,28 @@ __host__ __device__ constexpr int64_t row_bytes() {
     return (int64_t)NS * 64 * (int64_t)sizeof(JT);
   }
 }
+// The same for a slot count known at run time.
+template <typename JT>
+__host__ __device__ inline int64_t row_bytes_rt(int ns) {
+  if constexpr (sizeof(JT) == 1) {
+    return (int64_t)((ns + 3) / 4) * 256;
+  } else {
+    return (int64_t)ns * 64 * (int64_t)sizeof(JT);
+  }
+}
 
-// Value of element (slot m, lane) of a coupling row.
+// Value of element (slot m, lane) of a coupling row.  ns (wave-uniform,
+// <= NS): slots the row holds; slots m >= ns read as 0 and are not loaded.
 template <typename JT, typename FT, int NS>
 struct RowLoad {
   FT x[NS];
-  __device__ __forceinline__ void load(const char* __restrict__ row, int lane) {
+  __device__ __forceinline__ void load(const char* __restrict__ row, int lane, int ns = NS) {
     if constexpr (sizeof(JT) == 1) {
       constexpr int NQ = (NS + 3) / 4;
       uint32_t w[NQ];
       const uint32_t* p = reinterpret_cast<const uint32_t*>(row);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) w[q] = p[q * 64 + lane];
+      for (int q = 0; q < NQ; ++q) w[q] = (4 * q < ns) ? p[q * 64 + lane] : 0u;
 #pragma unroll
       for (int m = 0; m < NS; ++m) {
         const int32_t b = ((int32_t)(w[m >> 2] << (24 - 8 * (m & 3)))) >> 24;
@@ -88,7 +98,7 @@ struct RowLoad {
     } else {
       const JT* p = reinterpret_cast<const JT*>(row);
 #pragma unroll
-      for (int m = 0; m < NS; ++m) x[m] = (FT)p[m * 64 + lane];
+      for (int m = 0; m < NS; ++m) x[m] = (m < ns) ? (FT)p[m * 64 + lane] : (FT)0;
     }
   }
 };
@@ -111,23 +121,27 @@ __device__ __forceinline__ int32_t fma_ft<int32_t>(int32_t a, int32_t b, int32_t
 // F += c * row  (c = +-2, so c * J is exact and each field sees one rounding).
 // FP64 fields of 64 slots fill 128 VGPRs; a whole FP64 row in flight would take
 // the other 128, so such rows are applied in two halves of 32 slots.
+// ns (wave-uniform, <= NS): slots the row holds; fields of the other slots
+// (columns past the problem) receive zero updates.
 template <typename JT, typename FT, int NS>
 __device__ __forceinline__ void row_axpy(FieldVec<FT, NS>& F, const char* __restrict__ row,
-                                         int lane, FT c) {
+                                         int lane, FT c, int ns = NS) {
   constexpr bool halves = sizeof(FT) == 8 && sizeof(JT) == 8 && NS > 32;
   if constexpr (halves) {
     const JT* p = reinterpret_cast<const JT*>(row);
 #pragma unroll
     for (int m0 = 0; m0 < NS; m0 += 32) {
+      if (m0 >= ns) break;
       FT x[32];
 #pragma unroll
-      for (int m = 0; m < 32; ++m) x[m] = (FT)p[(m0 + m) * 64 + lane];
+      for (int m = 0; m < 32; ++m) x[m] = (m0 + m < ns) ? (FT)p[(m0 + m) * 64 + lane] : (FT)0;
 #pragma unroll
       for (int m = 0; m < 32; ++m) F.set(m0 + m, fma_ft<FT>(c, x[m], F.get(m0 + m)));
     }
   } else {
     RowLoad<JT, FT, NS> r;
-    r.load(row, lane);
+    r.load(row, lane, ns);
+    // slots past ns read as 0: their fields (columns past the problem) stay as they are
 #pragma unroll
     for (int m = 0; m < NS; ++m) F.set(m, fma_ft<FT>(c, r.x[m], F.get(m)));
   }
@@ -136,20 +150,21 @@ __device__ __forceinline__ void row_axpy(FieldVec<FT, NS>& F, const char* __rest
 // F = h + sum_k s_k J[k][.], k = 0..n-1 in order (one rounding per term).
 // `down` holds the lane's spin bits; the wave-uniform spin of k comes from a
 // ballot of slot k>>6.
+// ns (wave-uniform, <= NS): the problem's slot count (rows of ns slots).
 template <typename JT, typename FT, int NS>
 __device__ __forceinline__ void init_fields_seq(FieldVec<FT, NS>& F, const char* __restrict__ J,
                                                 const FT* __restrict__ h, uint64_t down,
-                                                int n, int lane) {
+                                                int n, int lane, int ns = NS) {
 #pragma unroll
-  for (int m = 0; m < NS; ++m) F.set(m, h[m * 64 + lane]);
-  constexpr int64_t RB = row_bytes<JT, NS>();
-  for (int q = 0; q < NS; ++q) {
+  for (int m = 0; m < NS; ++m) F.set(m, (m < ns) ? h[m * 64 + lane] : (FT)0);
+  const int64_t RB = (ns == NS) ? row_bytes<JT, NS>() : row_bytes_rt<JT>(ns);
+  for (int q = 0; q < ns; ++q) {
     const uint64_t bal = ballot64((down >> q) & 1ull);
     const int kend = min(64, n - 64 * q);
     for (int l2 = 0; l2 < kend; ++l2) {
       const int k = 64 * q + l2;
       const FT c = ((bal >> l2) & 1ull) ? (FT)(-1) : (FT)1;
-      row_axpy<JT, FT, NS>(F, J + (int64_t)k * RB, lane, c);
+      row_axpy<JT, FT, NS>(F, J + (int64_t)k * RB, lane, c, ns);
     }
   }
 }

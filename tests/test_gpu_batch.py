@@ -41,7 +41,7 @@ def test_batch_bitexact_per_problem(prec):
     inits = [H.random_pm1(R, E.shape[0] - 1, 40 + k) for k, E in enumerate(Es)]
     with capi.BatchContext.create(Es, R, PREC[prec]) as ctx:
         info = ctx.info()
-        assert info["kernel"] == capi.KERNEL_BATCH_WAVE
+        assert info["kernel"] == capi.KERNEL_BATCH_FUSED
         assert info["num_problems"] == len(Es)
         ctx.set_spins(inits)
         ctx.run_batch(betas, mcs, 2024, 9)
@@ -115,3 +115,34 @@ def test_batch_argument_errors():
     with capi.Context.dense(Es[0], 2) as d:
         rc = capi.lib().ojmi_run_batch(d.h, None, None, 0, 0, 0)
         assert rc == capi.EINVAL
+
+
+@pytest.mark.parametrize("mode,waves", [(0, 4), (2, 1), (2, 2), (2, 4)])
+def test_batch_launch_modes_agree(mode, waves):
+    """One launch of all problems largest first (each at its own slot count
+    inside the widest slot capacity), the persistent launch (per-XCD
+    longest-first queues, any wave may run any replica) and one launch per
+    slot class give the same spins bit for bit; 200 problems so that every
+    queue holds many items and stealing happens at the end."""
+    ns = H.qubo_batch_sizes(200, 0xD0D)
+    ns = np.minimum(ns, 700)
+    Es = [H.qubo_to_scaled_ising(H.qubo_upper(int(n), 0xD0E + k))[0] for k, n in enumerate(ns)]
+    R, L = 3, 4
+    betas = per_problem_betas(len(Es), L, 9)
+    mcs = np.ones(L, np.int64)
+    inits = [H.random_pm1(R, E.shape[0] - 1, 70 + k) for k, E in enumerate(Es)]
+    out = []
+    kernels = {0: capi.KERNEL_BATCH_FUSED, 1: capi.KERNEL_BATCH_WAVE, 2: capi.KERNEL_BATCH_PERSIST}
+    for m in (mode, 1):
+        with capi.BatchContext.create(Es, R, capi.PREC_INT32) as ctx:
+            ctx.set_option("batch_mode", m)
+            if m == 2:
+                ctx.set_option("batch_waves", waves)
+            assert ctx.info()["kernel"] == kernels[m]
+            ctx.set_spins(inits)
+            ctx.run_batch(betas, mcs, 321, 0)
+            out.append(ctx.get_spins())
+            flips = ctx.info()["last_flips"]
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a, b)
+    assert flips > 0
