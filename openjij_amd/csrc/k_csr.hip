@@ -19,6 +19,7 @@
 // four per lane up front into LDS (counter (i>>2, t, r, 0), word i&3), or, in
 // the workgroup form, the thresholds computed from them.
 #include "ojmi_kernels.h"
+#include "ojmi_tuned.h"
 #include "ojmi_device.h"
 
 namespace ojmi {
@@ -308,16 +309,10 @@ int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int 
   const size_t tb = ftype == FT_F32 ? 4 : 8;
   if (csr_wg_lds_bytes(n, nlev, tb) > CSR_WG_LDS_MAX) return 1;
   if (requested > 0) return requested;
-  // measured on 3D EA L=16, 200 sweeps, W = 1 / 2 / 4 / 8 (ms):
-  //   INT32  R = 512: 21.2 / 18.1 / 14.1 / 16.4   R = 1024: 25.9 / 18.9 / 18.8 / 29.3
-  //          R = 2048: 43.7 / 41.8 / 35.5 / 52.2
-  //   FP32   R = 512: 18.6 / 15.6 / 12.5 / 15.2   R = 1024: 20.7 / 16.4 / 16.5 / 25.2
-  //          R = 2048: 37.0 / 32.2 / 29.1 / 46.2
-  //   FP64   R = 512: 19.7 / 18.8 / 19.8 / 30.1   R = 1024: 25.0 / 22.3 / 31.0 / 58.8
-  //          R = 2048: 41.1 / 48.5 / 59.1 / 115.6
-  // (FP64 fields, values and thresholds take twice the registers and LDS.)
-  if (ftype == FT_F64) return R <= 1024 ? 2 : 1;
-  return 4;
+  // the autotuner's table (openjij_amd/tune.py, ojmi_tuned.h)
+  for (const tuned::CsrShape& e : tuned::kCsr)
+    if (e.ftype == ftype && R <= e.r_max) return e.waves;
+  return ftype == FT_F64 ? 1 : 4;
 }
 
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hipStream_t st) {

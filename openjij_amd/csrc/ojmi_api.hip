@@ -13,6 +13,7 @@
 #include "../../include/ojmi.h"
 #include "ojmi_kernels.h"
 #include "ojmi_persist.h"
+#include "ojmi_tuned.h"
 #include "ojmi_wave.h"
 
 using namespace ojmi;
@@ -767,11 +768,21 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
     delete c;
     return rc;
   }
-  if (c->M == 1 && c->ftype == FT_F32 && c->R >= 8 && c->family == OJMI_KERNEL_DENSE_WAVE) {
-    // column-owned fields with 4 replicas per workgroup where the row count
-    // allows it, else rows staged in LDS for 8 replicas
-    c->group = colown_supported((int)c->NS, 4) ? 4 : 8;
-    c->family = OJMI_KERNEL_DENSE_SHARED;
+  if (c->M == 1 && (c->ftype == FT_F32 || c->ftype == FT_I32) &&
+      c->family == OJMI_KERNEL_DENSE_WAVE) {
+    // workgroup shape from the autotuner's table (openjij_amd/tune.py,
+    // ojmi_tuned.h); replicas share coupling rows only when there are at
+    // least two workgroups' worth of them
+    for (const tuned::DenseShape& e : tuned::kDense) {
+      if (e.ftype != c->ftype || e.slots != c->NS || c->R > e.r_max) continue;
+      if (e.group > 1 && c->R >= 2 * e.group && c->R >= 8) {
+        c->group = e.group;
+        c->colown = e.colown;
+        c->xcd_window = e.xcd_window;
+        c->family = OJMI_KERNEL_DENSE_SHARED;
+      }
+      break;
+    }
   }
   *out = c;
   return OJMI_OK;
@@ -1125,6 +1136,7 @@ int ojmi_create_batch(int64_t num_problems, const int64_t* n, const double* J_ex
     return rc;
   }
   c->family = OJMI_KERNEL_BATCH_FUSED;
+  c->batch_mode = 0;
   // software queues: whole problems to 8 per-XCD queues in LPT order,
   // replicas consecutive
   if ((double)K * (double)num_replicas <= (double)INT32_MAX) {
@@ -1150,6 +1162,11 @@ int ojmi_create_batch(int64_t num_problems, const int64_t* n, const double* J_ex
       delete c;
       return rc;
     }
+  }
+  // default launch from the autotuner's table
+  if (tuned::kBatchMode == 1 || (tuned::kBatchMode == 2 && c->d_q_items)) {
+    c->batch_mode = tuned::kBatchMode;
+    c->family = tuned::kBatchMode == 1 ? OJMI_KERNEL_BATCH_WAVE : OJMI_KERNEL_BATCH_PERSIST;
   }
   *out = c;
   return OJMI_OK;

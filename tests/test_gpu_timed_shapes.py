@@ -1,8 +1,9 @@
 """The exact kernel instances bench.py times, under bit-exact parity.
 
-* Dense SK N=4096, FP32, default context (4 replicas per workgroup, column-
-  owned fields, 16 slots per wave chunk): a few sweeps of 8 replicas against
-  the FP32 replay.
+* Dense SK N=4096, FP32: the launch shape a 2048-replica context picks
+  (the autotuner's table; 4 replicas per workgroup with column-owned fields,
+  16 slots per wave chunk, as tuned on MI355X), run for a few sweeps of 8
+  replicas against the FP32 replay.
 * 4096 uneven QUBOs, 16 reads each: 128 problems drawn from the same size
   distribution, QUBO -> Ising with the exact x4 scaling, so that the
   workgroup order maps whole rounds of problems onto XCDs (non-identity
@@ -25,10 +26,15 @@ def test_config_e_kernel_shape_bitexact():
     beta = H.geometric_betas(0.1, 10.0, 500)[:L]
     mcs = np.ones(L, np.int64)
     init = H.random_pm1(R, 4096, 0xE1)
+    with capi.Context.dense(E, 2048, 1, 0.0, capi.PREC_FP32) as big:
+        shape = big.info()  # the launch shape config E's context runs with
+    assert shape["kernel"] in (capi.KERNEL_DENSE_COLOWN, capi.KERNEL_DENSE_SHARED,
+                               capi.KERNEL_DENSE_WAVE)
     with capi.Context.dense(E, R, 1, 0.0, capi.PREC_FP32) as ctx:
+        ctx.set_option("group", shape["group"])
+        ctx.set_option("colown", 1 if shape["kernel"] == capi.KERNEL_DENSE_COLOWN else 0)
         info = ctx.info()
-        assert info["kernel"] == capi.KERNEL_DENSE_COLOWN
-        assert info["group"] == 4
+        assert (info["kernel"], info["group"]) == (shape["kernel"], shape["group"])
         ctx.set_spins(init)
         ctx.run(beta, mcs, 1000, 0)
         got = ctx.get_spins()
