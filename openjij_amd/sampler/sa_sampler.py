@@ -24,6 +24,8 @@ from openjij_amd.sampler.sampler import (
     to_vartype_states,
 )
 
+FP32_REFRESH_EVERY = 16  # sweeps between matrix-core field refreshes on the FP32 path
+
 _PRECISIONS = {
     None: 0, "auto": 0, "fp64": 1, "float64": 1, "fp32": 2, "mixed": 2, "float32": 2,
     "int32": 3, "exact": 3,
@@ -248,6 +250,13 @@ class SASampler(BaseSampler):
             exec_us = np.zeros(R)
         elif reinitialize_state:
             eng = cxxjij.engine.Engine(graph, R, 1, 0.0, _precision_code(precision))
+            if _precision_code(precision) == 2:
+                # FP32 fields refreshed on the matrix cores every 16 sweeps
+                # (dense systems; sparse ones keep their carried fields)
+                try:
+                    eng.set_option("refresh_every", FP32_REFRESH_EVERY)
+                except RuntimeError:
+                    pass
             eng.set_spins(init)
             eng.run(beta, None, mcs, seed_v, 0)
             spins = eng.solution()
