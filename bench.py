@@ -518,6 +518,8 @@ def main():
     ap.add_argument("--xcd-window", type=int, default=None)
     ap.add_argument("--colown", type=int, default=None)
     ap.add_argument("--refresh-every", type=int, default=None)
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="any ojmi_set_option key for the head config's fast-path context")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; replicas would shard across GPUs "
@@ -537,6 +539,9 @@ def main():
         v = getattr(args, key)
         if v is not None:
             opts[key] = v
+    for kv in args.option:
+        k, _, v = kv.partition("=")
+        opts[k] = int(v)
     head_cfg = args.config or "E"
     head = run_config(head_cfg, args.steps, args.warmup, not args.no_cpu, not args.no_plain, opts,
                       args.sweeps, args.replicas, not args.no_tts)

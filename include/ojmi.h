@@ -183,6 +183,12 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *                    launch pulling (problem, replica) items from per-XCD
  *                    longest-first queues; results are identical
  *   "batch_waves":   batch contexts, batch_mode 2: waves per workgroup (1..4)
+ *   "lds_rows":      one-wave-per-replica dense contexts of <= 256 slots' worth
+ *                    of rows (NS <= 4, rows <= 64 KiB): 1 (default) copies the
+ *                    coupling rows to LDS once per workgroup and reads flipped
+ *                    rows from there, 0 reads them from L2
+ *   "lds_rows_wpb":  replicas per workgroup of that form (0 = by R: 1 up to
+ *                    256 replicas, then up to 4)
  *   "csr_waves":     CSR contexts of degree <= 16: waves sharing one replica's
  *                    sweep (1, 2, 4, 8; 0 = chosen by the replica count)
  *   "update_method": acceptance rule of classical contexts (dense, CSR,

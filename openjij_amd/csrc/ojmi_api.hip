@@ -87,6 +87,8 @@ struct ojmi_ctx {
   int lds_budget = 65536;
   int ring_rows_opt = 0, round_rows_opt = 0, ring_rows_used = 0, lds_used = 0;
   int colown = 1;  // shared family: 1 = column-owned kernel when the shape allows
+  int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
+  int lds_rows_wpb = 0;  // replicas per workgroup of that form (0: by R)
   void* dJnat = nullptr;   // natural-layout couplings when dJ is packed
   uint32_t* dprog = nullptr;
   int64_t prog_cap = 0;
@@ -652,7 +654,21 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     c->lds_used = (int)lds_kernel_bytes((int)c->Npad, c->ftype == FT_F64 ? 8 : 4);
     return launch_dense_lds(c->ftype, c->jtype, a, (int)c->Npad, c->st);
   }
-  if (c->family == OJMI_KERNEL_DENSE_WAVE) return launch_dense_wave(c->ftype, c->jtype, a, c->NS, c->st);
+  if (c->family == OJMI_KERNEL_DENSE_WAVE) {
+    const int jb = c->jtype == JT_I8 ? 1 : (c->jtype == JT_F64 ? 8 : 4);
+    const size_t lds = wave_lds_bytes((int)c->n, c->NS, jb);
+    const bool rows_in_lds = c->lds_rows != 0 && c->NS <= 4 && lds <= (size_t)65536 &&
+                             lds <= (size_t)c->lds_budget;
+    if (rows_in_lds) {
+      // fewer replicas per workgroup when there are few: more CUs, one LDS copy each
+      const int wpb = c->lds_rows_wpb > 0 ? c->lds_rows_wpb
+                                          : (int)std::min<int64_t>(4, std::max<int64_t>(1, (c->R + 255) / 256));
+      c->lds_used = (int)lds;
+      return launch_dense_wave_lds(c->ftype, c->jtype, a, c->NS, wpb, c->st);
+    }
+    c->lds_used = 0;
+    return launch_dense_wave(c->ftype, c->jtype, a, c->NS, c->st);
+  }
 
   // workgroup-shared families
   SharedArgs sa{};
@@ -1278,6 +1294,16 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   if (k == "colown") {
     if (value != 0 && value != 1) return fail(OJMI_EINVAL, "colown must be 0 or 1");
     c->colown = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "lds_rows") {
+    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "lds_rows must be 0 or 1");
+    c->lds_rows = value ? 1 : 0;
+    return OJMI_OK;
+  }
+  if (k == "lds_rows_wpb") {
+    if (value < 0 || value > 4) return fail(OJMI_EINVAL, "lds_rows_wpb must be 0..4");
+    c->lds_rows_wpb = (int)value;
     return OJMI_OK;
   }
   if (k == "round_rows") {

@@ -105,6 +105,11 @@ hipError_t launch_dense_colown(int ftype, int jtype, const SharedArgs& a, int NS
                                hipStream_t st);
 
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
+// The wave kernel with the coupling rows staged in LDS (NS <= 4): bytes of
+// LDS per workgroup, and the launch with wpb waves (replicas) per workgroup.
+size_t wave_lds_bytes(int n, int NS, int jbytes);
+hipError_t launch_dense_wave_lds(int ftype, int jtype, const DenseArgs& a, int NS, int wpb,
+                                 hipStream_t st);
 hipError_t launch_batch_wave(int ftype, int jtype, const BatchArgs& b, int NS, hipStream_t st);
 // Classical dense systems too large for register-resident fields (k_lds.hip):
 // natural coupling rows of Npad = 64 * ceil(n / 64), fields in LDS.

@@ -101,6 +101,35 @@ def test_config_a_sk256_pm1_replay():
         np.testing.assert_array_equal(en, orc.energy_exact(E, got))
 
 
+@pytest.mark.parametrize("n,prec,R", [(256, "int32", 64), (256, "int32", 300), (200, "int32", 5),
+                                      (64, "fp64", 9), (64, "fp32", 70), (40, "fp64", 3)])
+@pytest.mark.parametrize("wpb", [0, 1, 3, 4])
+def test_dense_wave_rows_in_lds_bitexact(n, prec, R, wpb):
+    """The wave kernel with the coupling rows staged in LDS (rows of up to
+    64 KiB in total), for every replicas-per-workgroup choice, against the
+    replay and against the same context reading its rows from L2."""
+    E = H.sk_pm1(n, 0x1D5 + n) if prec == "int32" else H.sk_gauss(n, 0x1D5 + n, field=True)
+    beta, mcs = sched(60, 0.1, 6.0)
+    init = H.random_pm1(R, n, 21)
+    out = {}
+    for lds in (1, 0):
+        with capi.Context.dense(E, R, 1, 0.0, PREC[prec]) as ctx:
+            ctx.set_option("lds_rows", lds)
+            ctx.set_option("lds_rows_wpb", wpb)
+            ctx.set_spins(init)
+            ctx.run(beta, mcs, 0x1D5, 0)
+            info = ctx.info()
+            assert info["kernel"] == capi.KERNEL_DENSE_WAVE
+            rb = ((n + 63) // 64 + 3) // 4 * 256 if prec == "int32" else \
+                (n + 63) // 64 * 64 * (8 if prec == "fp64" else 4)
+            assert info["lds_bytes"] == (n * rb if lds else 0)
+            out[lds] = ctx.get_spins()
+    want = orc.replay_dense(prec, H.symmetric_ext(E), init, 0x1D5, 0,
+                            orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(out[1], want)
+    np.testing.assert_array_equal(out[0], want)
+
+
 # --- CSR --------------------------------------------------------------------
 @pytest.mark.parametrize("L,R,sweeps", [(3, 4, 30), (6, 16, 20), (16, 4, 6), (4, 600, 8), (3, 2100, 6)])
 def test_csr_ea3d_int_bitexact(L, R, sweeps):
