@@ -171,7 +171,8 @@ __device__ __forceinline__ void load_chunk(FT (&x)[CS], const JT* __restrict__ p
 // RULE: acceptance rule (RULE_METROPOLIS / RULE_HEAT_BATH) fixed at compile
 // time, so the Metropolis decision code has no rule branch.
 template <typename FT, typename JT, int NS, int G, int RULE>
-__global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
+__global__ __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(2)))
+void k_dense_colown(SharedArgs a) {
   typedef typename ColThresh<FT>::type TT;
   constexpr int CS = NS / G;
   static_assert(CS >= 1 && CS * G == NS, "NS must be a multiple of G");
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
   static_assert(CS % VEC == 0 && 64 % VEC == 0, "chunk layout");
   constexpr int Npad = 64 * NS;
   constexpr int TILE_BYTES = 64 * 64 * (int)sizeof(JT);
-  constexpr int PF = (CS >= 8) ? 2 : ((CS >= 4) ? 4 : 8);  // rows per prefetch group
+  constexpr int PF = (CS >= 16) ? 2 : ((CS >= 4) ? 32 / CS : 8);  // rows per prefetch group
 
   extern __shared__ __align__(16) char smem[];
   char* sD = smem;                                                   // 1 tile
@@ -212,7 +213,11 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     }
   }
 
-  FieldVec<FT, NS> F;
+  // Fields as NS separate values, every index static: a replica's row update
+  // under its flip test then writes its registers in place (a vector type
+  // would tie the NS values to one register tuple, which the conditional
+  // updates would copy around).
+  FT F[NS];
   if (a.F_in) {
     const FT* fi = reinterpret_cast<const FT*>(a.F_in);
 #pragma unroll
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
       const int rg = (rbase + g < a.R) ? rbase + g : 0;
 #pragma unroll
       for (int s = 0; s < CS; ++s)
-        F.set(g * CS + s, fi[(int64_t)rg * Npad + slot_col<VEC>(col0, s, lane)]);
+        F[g * CS + s] = fi[(int64_t)rg * Npad + slot_col<VEC>(col0, s, lane)];
     }
   } else {
     // F = h + sum_k s_k J[k][.], k in order, for every replica; the spins of
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     for (int s = 0; s < CS; ++s) {
       const FT hv = h[slot_col<VEC>(col0, s, lane)];
 #pragma unroll
-      for (int g = 0; g < G; ++g) F.set(g * CS + s, hv);
+      for (int g = 0; g < G; ++g) F[g * CS + s] = hv;
     }
     for (int k = 0; k < n; ++k) {
       FT x[CS];
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
       for (int g = 0; g < G; ++g) {
         const FT c = ((bits >> g) & 1u) ? (FT)(-1) : (FT)1;
 #pragma unroll
-        for (int s = 0; s < CS; ++s) F.set(g * CS + s, fma_ft<FT>(c, x[s], F.get(g * CS + s)));
+        for (int s = 0; s < CS; ++s) F[g * CS + s] = fma_ft<FT>(c, x[s], F[g * CS + s]);
       }
     }
     __syncthreads();
@@ -297,13 +302,28 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
         const bool holds = lane >= lb && lane < lb + 64 / VEC;
         // ---- 0: the owner publishes block k's fields and spin states
         if (w == ow && holds) {
+          // qb is wave-uniform: scalar branches pick the static slots
+          FT pub[G][VEC];
+#pragma unroll
+          for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) pub[g][e] = F[g * CS + e];
+#pragma unroll
+          for (int cq = 1; cq < CS / VEC; ++cq) {
+            if (cq != qb) continue;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+#pragma unroll
+              for (int e = 0; e < VEC; ++e) pub[g][e] = F[g * CS + VEC * cq + e];
+          }
+          FT* pf = sF + VEC * (lane - lb);
+          uint8_t* pd = sDn + VEC * (lane - lb);
 #pragma unroll
           for (int g = 0; g < G; ++g) {
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-              const int t = VEC * (lane - lb) + e;
-              sF[g * 64 + t] = F.dyn(g * CS + VEC * qb + e);
-              sDn[g * 64 + t] = (uint8_t)((down >> (g * CS + VEC * qb + e)) & 1ull);
+              pf[g * 64 + e] = pub[g][e];
+              pd[g * 64 + e] = (uint8_t)((down >> (g * CS + VEC * qb + e)) & 1ull);
             }
           }
         }
@@ -400,7 +420,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
                 const FT c = coef2<FT>((sg[g] >> u) & 1ull);
 #pragma unroll
                 for (int s = 0; s < CS; ++s)
-                  F.set(g * CS + s, fma_ft<FT>(c, x[p][s], F.get(g * CS + s)));
+                  F[g * CS + s] = fma_ft<FT>(c, x[p][s], F[g * CS + s]);
               }
             }
           }
@@ -456,7 +476,7 @@ __global__ __launch_bounds__(64 * G) void k_dense_colown(SharedArgs a) {
     if (a.F_out) {
       FT* fo = reinterpret_cast<FT*>(a.F_out) + (int64_t)(rbase + g) * Npad;
 #pragma unroll
-      for (int s = 0; s < CS; ++s) fo[slot_col<VEC>(col0, s, lane)] = F.get(g * CS + s);
+      for (int s = 0; s < CS; ++s) fo[slot_col<VEC>(col0, s, lane)] = F[g * CS + s];
     }
   }
 }
