@@ -339,6 +339,10 @@ py::object csr_to_scipy_symmetric(const CSRGraph& g) {
 // systems: host state plus a lazily created one-replica GPU context
 struct Ctx {
   ojmi_ctx* p = nullptr;
+  // the spins the device holds (after the last upload or download): a run
+  // whose state is unchanged keeps the device's fields, as the reference
+  // system keeps dE from one Algorithm_run to the next
+  std::vector<int8_t> held;
   ~Ctx() {
     if (p) ojmi_destroy(p);
   }
@@ -443,13 +447,17 @@ void push(ClassicalSystem<G>& s, ojmi_ctx* c) {
   for (size_t i = 0; i < s.num_spins; ++i) v[i] = (int8_t)(s.spin[i] * d > 0 ? 1 : -1);
   s.spin[s.num_spins] = 1;
   for (size_t i = 0; i < s.num_spins; ++i) s.spin[i] = v[i];
-  check(ojmi_set_spins(c, v.data()));
+  if (s.ctx->held != v) {
+    check(ojmi_set_spins(c, v.data()));
+    s.ctx->held = v;
+  }
 }
 template <typename G>
 void pull(ClassicalSystem<G>& s, ojmi_ctx* c) {
   std::vector<int8_t> v(s.num_spins);
   check(ojmi_get_spins(c, v.data()));
   for (size_t i = 0; i < s.num_spins; ++i) s.spin[i] = v[i];
+  s.ctx->held = v;
 }
 template <typename G>
 void push(TransverseSystem<G>& s, ojmi_ctx* c) {
@@ -457,7 +465,10 @@ void push(TransverseSystem<G>& s, ojmi_ctx* c) {
   std::vector<int8_t> v(n * M);
   for (size_t t = 0; t < M; ++t)
     for (size_t i = 0; i < n; ++i) v[t * n + i] = (int8_t)(s.trotter[t][i] > 0 ? 1 : -1);
-  check(ojmi_set_spins(c, v.data()));
+  if (s.ctx->held != v) {
+    check(ojmi_set_spins(c, v.data()));
+    s.ctx->held = v;
+  }
 }
 template <typename G>
 void pull(TransverseSystem<G>& s, ojmi_ctx* c) {
@@ -466,6 +477,7 @@ void pull(TransverseSystem<G>& s, ojmi_ctx* c) {
   check(ojmi_get_spins(c, v.data()));
   for (size_t t = 0; t < M; ++t)
     for (size_t i = 0; i < n; ++i) s.trotter[t][i] = v[t * n + i];
+  s.ctx->held = v;
 }
 
 // ---------------------------------------------------------------------------

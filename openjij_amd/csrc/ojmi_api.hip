@@ -96,19 +96,22 @@ struct ojmi_ctx {
   // kernel-layout couplings
   void* dJ = nullptr;
   void* dh = nullptr;
-  // FP64 natural padded couplings for energies (Npad x Npad) and h
+  // FP64 natural padded couplings for energies (Npad x Npad) and h (dense)
   double* dJ64 = nullptr;
   double* dh64 = nullptr;
-  // CSR level-order data
+  // CSR level-order data; FP64 neighbour values and fields at level-order
+  // positions for energies (the sweep's own arrays when those are FP64)
   int32_t *d_order = nullptr, *d_lvl = nullptr, *d_deg = nullptr, *d_ell = nullptr;
   void* d_ellval = nullptr;
+  double* d_ellv64 = nullptr;
+  double* d_hpos64 = nullptr;
   int nlev = 0, D = 0;
 
   int8_t* dspins = nullptr;
   double* dE = nullptr;    // [C]
   double* ddrift = nullptr;
   void* dF = nullptr;      // [R][Npad] fields (FP32 or int32): MFMA init / refresh, sweep output
-  bool fields_valid = false;  // dF holds the fields of the current spins (FP32 path)
+  bool fields_valid = false;  // dF holds the fields of the current spins (FP32 / FP64 dense)
   bool energies_valid = false;  // dE / ddrift hold the current spins' energies and drift
   // matrix-core operands: bf16 hi/lo couplings (FP32 path), spins zero-padded
   // to [C][Npad] as int8 and bf16, energy partials
@@ -157,7 +160,10 @@ struct ojmi_ctx {
   ~ojmi_ctx() {
     if (dJ == dJ64) dJ = nullptr;
     if (dh == dh64) dh = nullptr;
+    if (d_ellv64 == d_ellval) d_ellv64 = nullptr;
+    if (d_hpos64 == dh) d_hpos64 = nullptr;
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
+                    d_ellv64, d_hpos64,
                     dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
                     d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
                     dJhi, dJlo, dS8, dSbf, dpart, dstats, d_q_items, d_q_begin, d_q_head,
@@ -545,11 +551,25 @@ int end_stats(ojmi_ctx* c) {
 // [R][Npad] field buffer (4-byte elements: FP32 or int32 fields).
 int ensure_fields(ojmi_ctx* c) {
   if (c->dF) return OJMI_OK;
-  float* f;
-  int rc = dalloc(&f, (size_t)(c->R * c->Npad));
-  if (rc) return rc;
-  c->dF = f;
+  int rc;
+  if (c->ftype == FT_F64) {
+    double* f;
+    if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
+    c->dF = f;
+  } else {
+    float* f;
+    if ((rc = dalloc(&f, (size_t)(c->R * c->Npad)))) return rc;
+    c->dF = f;
+  }
   return OJMI_OK;
+}
+
+// The classical dense families carry real-valued fields from one run to the
+// next (F_out, then F_in while the spins are unchanged): a run split into
+// pieces, or one sweep per call with a callback, follows the trajectory of a
+// single run.  Integer fields are rebuilt exactly instead.
+bool carries_fields(const ojmi_ctx* c) {
+  return c->kind == 0 && c->M == 1 && (c->ftype == FT_F32 || c->ftype == FT_F64);
 }
 
 // Zero-padded spins for the matrix-core kernels: int8 [R*M][Npad] (fields of
@@ -625,7 +645,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.stats = c->dstats;
     return launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   }
-  void* F_out = (c->ftype == FT_F32 && c->M == 1) ? c->dF : nullptr;
+  void* F_out = carries_fields(c) ? c->dF : nullptr;
   DenseArgs a{};
   a.J = c->dJ;
   a.h = c->dh;
@@ -981,22 +1001,22 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
     delete c;
     return fail(OJMI_EDEVICE, "hipMemcpy of CSR values failed");
   }
-  // FP64 dense copy for energies when it fits (n <= 16384), else CSR energies
-  if (n <= 16384) {
-    const int64_t Np = c->Npad;
-    std::vector<double> J64((size_t)(Np * Np), 0.0), h64((size_t)Np, 0.0);
-    for (int64_t i = 0; i < n; ++i) {
-      for (auto& p : adj[i]) J64[i * Np + p.first] = p.second;
-      h64[i] = h[i];
-    }
-    if ((rc = dalloc(&c->dJ64, J64.size())) || (rc = dalloc(&c->dh64, h64.size()))) {
+  // energies run over the same ELL arrays with FP64 values (k_csr_energy);
+  // FP32 and int32 contexts keep an FP64 copy of the values (Dp x n)
+  if (c->ftype == FT_F64) {
+    c->d_ellv64 = reinterpret_cast<double*>(dv);
+    c->d_hpos64 = reinterpret_cast<double*>(dhb);
+  } else {
+    if ((rc = dalloc(&c->d_ellv64, ellv.size())) || (rc = dalloc(&c->d_hpos64, hpos.size()))) {
       delete c;
       return rc;
     }
-    if (hipMemcpy(c->dJ64, J64.data(), J64.size() * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->dh64, h64.data(), h64.size() * 8, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(c->d_ellv64, ellv.data(), ellv.size() * 8, hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(c->d_hpos64, hpos.data(), hpos.size() * 8, hipMemcpyHostToDevice) !=
+            hipSuccess) {
       delete c;
-      return fail(OJMI_EDEVICE, "hipMemcpy of FP64 couplings failed");
+      return fail(OJMI_EDEVICE, "hipMemcpy of FP64 CSR values failed");
     }
   }
   *out = c;
@@ -1455,7 +1475,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     const int64_t chunk = mfma_bf ? c->refresh_every : T;
     for (int64_t t = 0; t < T && e == hipSuccess; t += chunk) {
       const int64_t Tc = std::min<int64_t>(chunk, T - t);
-      const void* F_in = nullptr;
+      const void* F_in = (t == 0 && c->fields_valid && carries_fields(c)) ? c->dF : nullptr;
       if (mfma_i8 || mfma_bf) {
         if (mfma_bf) {
           if ((size_t)(2 * c->last_refreshes + 2) > c->ev_ref.size()) {
@@ -1475,7 +1495,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
       }
       e = launch_sweeps(c, t, Tc, seed, sweep_offset, F_in);
     }
-    c->fields_valid = (c->ftype == FT_F32);
+    c->fields_valid = carries_fields(c);
   }
   if (e != hipSuccess) return fail(OJMI_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
   OJMI_HIP(hipEventRecord(c->e1, c->st));
@@ -1519,6 +1539,11 @@ int ojmi_get_fields(ojmi_ctx* c, double* out) {
     OJMI_HIP(hipMemcpy(f.data(), c->dF, f.size() * 4, hipMemcpyDeviceToHost));
     for (int64_t r = 0; r < R; ++r)
       for (int64_t j = 0; j < n; ++j) out[r * n + j] = (double)f[(size_t)(r * Np + j)];
+  } else if (c->ftype == FT_F64) {
+    std::vector<double> f((size_t)(R * Np));
+    OJMI_HIP(hipMemcpy(f.data(), c->dF, f.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t r = 0; r < R; ++r)
+      for (int64_t j = 0; j < n; ++j) out[r * n + j] = f[(size_t)(r * Np + j)];
   } else {
     std::vector<int32_t> f((size_t)(R * Np));
     OJMI_HIP(hipMemcpy(f.data(), c->dF, f.size() * 4, hipMemcpyDeviceToHost));
@@ -1531,12 +1556,23 @@ int ojmi_get_fields(ojmi_ctx* c, double* out) {
 static int compute_energies(ojmi_ctx* c) {
   if (c->energies_valid) return OJMI_OK;
   const int64_t C = c->R * c->M;
+  if (c->d_ellv64) {  // sparse system on the CSR kernel
+    hipError_t e = launch_csr_energy(c->dspins, (int)c->n, (int)C, c->d_order, c->d_deg,
+                                     c->d_ell, c->d_ellv64, c->d_hpos64, c->dE, c->st);
+    if (e != hipSuccess)
+      return fail(OJMI_EDEVICE, std::string("CSR energy kernel: ") + hipGetErrorString(e));
+    OJMI_HIP(hipStreamSynchronize(c->st));
+    c->energies_valid = true;
+    return OJMI_OK;
+  }
   if (!c->dJ64) return fail(OJMI_ENOTSUP, "energies need the FP64 coupling copy");
   int rc;
   if ((rc = ensure_mfma_spins(c))) return rc;
   if (!c->dpart && (rc = dalloc(&c->dpart, 2 * energy_part_count((int)C, (int)c->Npad))))
     return rc;
-  const float* F32 = (c->fields_valid && c->dF && c->M == 1) ? (const float*)c->dF : nullptr;
+  const float* F32 = (c->fields_valid && c->dF && c->M == 1 && c->ftype == FT_F32)
+                         ? (const float*)c->dF
+                         : nullptr;
   hipError_t e = launch_pack_spins(c->dspins, (int)c->n, (int)C, (int)c->Npad, c->dS8, nullptr,
                                    c->st);
   if (e == hipSuccess)
@@ -1566,8 +1602,25 @@ int ojmi_energies(ojmi_ctx* c, double* out) {
 int ojmi_energies_static(ojmi_ctx* c, double* out) {
   if (!c || !out) return fail(OJMI_EINVAL, "null argument");
   if (c->kind == 2) return fail(OJMI_ENOTSUP, "ojmi_energies_static: batch contexts use ojmi_energies");
-  if (!c->dJ64) return fail(OJMI_ENOTSUP, "energies need the FP64 coupling copy");
   const int64_t C = c->R * c->M, Np = c->Npad;
+  if (c->d_ellv64) {  // sparse system: the CSR energy kernel needs only E[C]
+    if ((size_t)C * 8 > STATIC_SCRATCH_BYTES)
+      return fail(OJMI_ERANGE, "ojmi_energies_static: " + std::to_string(C) +
+                                   " configurations exceed the static buffer");
+    void* base = nullptr;
+    hipError_t e = static_scratch(c->device, &base);
+    if (e != hipSuccess)
+      return fail(OJMI_EDEVICE, std::string("static scratch lookup: ") + hipGetErrorString(e));
+    double* E = static_cast<double*>(base);
+    e = launch_csr_energy(c->dspins, (int)c->n, (int)C, c->d_order, c->d_deg, c->d_ell,
+                          c->d_ellv64, c->d_hpos64, E, c->st);
+    if (e != hipSuccess)
+      return fail(OJMI_EDEVICE, std::string("CSR energy kernel: ") + hipGetErrorString(e));
+    OJMI_HIP(hipMemcpyAsync(out, E, (size_t)C * 8, hipMemcpyDeviceToHost, c->st));
+    OJMI_HIP(hipStreamSynchronize(c->st));
+    return OJMI_OK;
+  }
+  if (!c->dJ64) return fail(OJMI_ENOTSUP, "energies need the FP64 coupling copy");
   const size_t parts = energy_part_count((int)C, (int)Np);
   const size_t s8 = (((size_t)(C * Np) + 255) / 256) * 256;
   const size_t need = s8 + 2 * parts * 8 + (size_t)C * 8;
@@ -1597,7 +1650,8 @@ int ojmi_energies_static(ojmi_ctx* c, double* out) {
 int ojmi_field_drift(ojmi_ctx* c, double* out) {
   if (!c || !out) return fail(OJMI_EINVAL, "null argument");
   *out = 0.0;
-  if (!(c->fields_valid && c->dF && c->M == 1 && c->kind == 0)) return OJMI_OK;
+  if (!(c->fields_valid && c->dF && c->M == 1 && c->kind == 0 && c->ftype == FT_F32))
+    return OJMI_OK;
   int rc = compute_energies(c);
   if (rc) return rc;
   std::vector<double> d((size_t)c->R);

@@ -152,6 +152,11 @@ hipError_t launch_fields_i8(const int8_t* S8, const int8_t* J8, const int32_t* h
 // given, drift[c] = max_j |F32[c][j] - (H_cj + h_j)|.  part_e / part_d hold
 // energy_part_count(C, Npad) doubles each.
 size_t energy_part_count(int C, int Npad);
+// Energies of C configurations of a sparse system from the level-order ELL
+// arrays (FP64 values and fields at level-order positions), E[C].
+hipError_t launch_csr_energy(const int8_t* spins, int n, int C, const int32_t* order,
+                             const int32_t* deg, const int32_t* ell, const double* val,
+                             const double* hpos, double* E, hipStream_t st);
 // Device address (on `device`) of the static scratch array of the
 // allocation-free energies, STATIC_SCRATCH_BYTES long; looked up once per
 // device with that device current.

@@ -254,6 +254,35 @@ def test_cxxjij_run_replays_on_the_oracle(kind):
 
 
 @pytest.mark.gpu
+def test_callback_run_with_real_couplings_equals_plain_run():
+    """Gaussian couplings (FP64 path): one sweep per device call with a
+    callback keeps the device's fields between sweeps, so the trajectory is
+    the plain run's, and both equal the replay of the oracle."""
+    n = 150
+    E = H.sk_gauss(n, 0xCB, field=True)
+    g = cj.graph.Dense(n)
+    for i in range(n):
+        g[i] = E[i, n]
+        for j in range(i + 1, n):
+            g[i, j] = E[i, j]
+    s0 = g.gen_spin(3)
+    tl = [(0.2, 4), (1.0, 3), (4.0, 5)]
+    a = cj.system.make_classical_ising(s0, g)
+    cj.algorithm.Algorithm_SingleSpinFlip_run(a, 77, tl)
+    b = cj.system.make_classical_ising(s0, g)
+    calls = []
+    cj.algorithm.Algorithm_SingleSpinFlip_run(b, 77, tl, lambda sys, beta: calls.append(beta))
+    assert len(calls) == 12
+    assert list(a.spin) == list(b.spin)
+    init = np.array(s0[:n], dtype=np.int8).reshape(1, n)
+    beta = np.array([0.2, 1.0, 4.0])
+    mcs = np.array([4, 3, 5], np.int64)
+    want = orc.replay_dense("fp64", H.symmetric_ext(E), init, 77, 0,
+                            orc.expand_classical(beta, mcs))
+    assert list(a.spin[:n]) == list(want[0].astype(float))
+
+
+@pytest.mark.gpu
 def test_sparse_system_equals_csr_system():
     s0 = fill_case1(cj.graph.Dense(8)).gen_spin(5)
     out = []

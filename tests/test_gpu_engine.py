@@ -90,6 +90,35 @@ def test_split_run_equals_single_run():
         np.testing.assert_array_equal(c1.get_spins(), c2.get_spins())
 
 
+@pytest.mark.parametrize("prec,n", [("fp64", 200), ("fp32", 300), ("fp32", 2048)])
+def test_split_run_real_couplings_carries_fields(prec, n):
+    """Real couplings: a run in three pieces (and one sweep per call) keeps
+    the carried fields between calls, so it follows the single run's
+    trajectory bit for bit, as the reference system keeps dE across calls."""
+    E = H.sk_gauss(n, 0x5B1 + n, field=True)
+    beta, mcs = sched(12, 0.1, 6.0)
+    init = H.random_pm1(5, n, 8)
+    with capi.Context.dense(E, 5, 1, 0.0, PREC[prec]) as c1, \
+            capi.Context.dense(E, 5, 1, 0.0, PREC[prec]) as c2, \
+            capi.Context.dense(E, 5, 1, 0.0, PREC[prec]) as c3:
+        for c in (c1, c2, c3):
+            c.set_spins(init)
+        c1.run(beta, mcs, 42, 0)
+        for a, b in ((0, 5), (5, 6), (6, 12)):
+            c2.run(beta[a:b], mcs[a:b], 42, a)
+        for t in range(12):
+            c3.run(beta[t:t + 1], mcs[t:t + 1], 42, t)
+        want = orc.replay_dense(prec, H.symmetric_ext(E), init, 42, 0,
+                                orc.expand_classical(beta, mcs))
+        np.testing.assert_array_equal(c1.get_spins(), want)
+        np.testing.assert_array_equal(c2.get_spins(), want)
+        np.testing.assert_array_equal(c3.get_spins(), want)
+        # new spins: the fields are rebuilt from them
+        c2.set_spins(init)
+        c2.run(beta, mcs, 42, 0)
+        np.testing.assert_array_equal(c2.get_spins(), want)
+
+
 def test_config_a_sk256_pm1_replay():
     """Dense SK N=256, J = +-1, 64 reads, 1000 sweeps, beta 0.1 -> 10."""
     E = H.sk_pm1(256, 0xA001)
@@ -146,6 +175,63 @@ def test_csr_ea3d_int_bitexact(L, R, sweeps):
     np.testing.assert_array_equal(got, want)
     for r in range(R):
         assert en[r] == orc.energy_exact_csr(rp, ci, v, n, got[r])
+
+
+def gauss_sparse_csr(n, deg, seed):
+    """Random sparse graph, Gaussian couplings and fields (not FP32-exact),
+    as the full symmetric (n+1)-row CSR with corner 1."""
+    rng = np.random.default_rng(seed)
+    pairs = set()
+    while len(pairs) < n * deg // 2:
+        i, j = rng.integers(0, n, 2)
+        if i != j:
+            pairs.add((min(i, j), max(i, j)))
+    rows = [[] for _ in range(n + 1)]
+    for i, j in sorted(pairs):
+        v = rng.normal() / 3.0
+        rows[i].append((j, v))
+        rows[j].append((i, v))
+    for i in range(n):
+        hv = rng.normal() / 5.0
+        rows[i].append((n, hv))
+        rows[n].append((i, hv))
+    rows[n].append((n, 1.0))
+    rp, ci, va = [0], [], []
+    for r in rows:
+        for j, v in sorted(r):
+            ci.append(j)
+            va.append(v)
+        rp.append(len(ci))
+    return np.array(rp, np.int64), np.array(ci, np.int64), np.array(va, float)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "int32"])
+def test_csr_energies_from_level_order_arrays(prec):
+    """Energies of sparse systems come from the CSR kernel's own ELL arrays
+    with FP64 values: exact energies of the original couplings for every
+    precision of the sweep (integer couplings equal, real ones to 1e-12)."""
+    if prec == "int32":
+        n, rp, ci, v = H.ea3d_csr(9, 0xE9)
+    else:
+        n = 3000
+        rp, ci, v = gauss_sparse_csr(n, 5, 0xE5)
+    R = 7
+    beta, mcs = sched(10, 0.1, 3.0)
+    init = H.random_pm1(R, n, 5)
+    with capi.Context.csr(n, rp, ci, v, R, precision=PREC[prec]) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_CSR_LEVEL
+        ctx.set_spins(init)
+        first = ctx.energies()
+        ctx.run(beta, mcs, 99, 0)
+        got = ctx.get_spins()
+        en = ctx.energies()
+    for r in range(R):
+        for e, s in ((first[r], init[r]), (en[r], got[r])):
+            ex = orc.energy_exact_csr(rp, ci, v, n, s)
+            if prec == "int32":
+                assert e == ex
+            else:
+                assert abs(e - ex) <= 1e-12 * (1 + abs(ex)), (r, e, ex)
 
 
 @pytest.mark.parametrize("waves", [1, 2, 4, 8])

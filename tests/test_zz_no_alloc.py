@@ -42,6 +42,24 @@ def test_static_energies_equal_allocating_entry_point_on_two_calls():
             assert np.all(np.abs(got - ex) <= 1e-12 * (1 + np.abs(ex)))
 
 
+def test_static_energies_of_sparse_contexts_on_two_calls():
+    """Sparse contexts on the CSR kernel: the static entry point runs the
+    level-order energy kernel into the static buffer."""
+    for L, R in ((8, 300), (4, 9)):  # larger first
+        n, rp, ci, v = H.ea3d_csr(L, 0xA110 + L)
+        init = H.random_pm1(R, n, L)
+        with capi.Context.csr(n, rp, ci, v, R, precision=capi.PREC_INT32) as ctx:
+            assert ctx.info()["kernel"] == capi.KERNEL_CSR_LEVEL
+            ctx.set_spins(init)
+            got = static_energies(ctx)
+            want = ctx.energies()
+            again = static_energies(ctx)
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(again, want)
+        for r in range(R):
+            assert got[r] == orc.energy_exact_csr(rp, ci, v, n, init[r])
+
+
 def test_static_energies_reject_contexts_beyond_the_scratch():
     E = H.sk_pm1(4096, 1)
     with capi.Context.dense(E, 4096, 1, 0.0, capi.PREC_FP64) as ctx:
