@@ -331,6 +331,10 @@ KERNEL_NAMES = {1: "k_dense_wave", 2: "k_sqa_wave", 3: "k_csr_level", 4: "k_dens
 def kernel_name(info):
     if info["kernel"] == 3 and info.get("waves_per_replica", 1) > 1:
         return "k_csr_wg"  # several waves per replica
+    if info["kernel"] == 1 and info.get("waves_per_replica", 1) > 1:
+        return "k_dense_wave_pipe"  # threshold helper waves
+    if info["kernel"] == 1 and info.get("lds_bytes", 0) > 0:
+        return "k_dense_wave_lds"  # coupling rows in LDS
     return KERNEL_NAMES.get(info["kernel"], str(info["kernel"]))
 
 

@@ -79,7 +79,8 @@ typedef struct ojmi_info {
   int32_t lds_bytes;      /* LDS per workgroup of the last launch            */
   int32_t lds_budget;     /* device LDS limit per workgroup                  */
   int64_t num_problems;   /* problems of a batch context (1 otherwise)       */
-  int32_t waves_per_replica; /* CSR: waves sharing one replica's sweep        */
+  int32_t waves_per_replica; /* CSR: waves sharing one replica's sweep;     */
+                             /* dense one-wave form: 1 + threshold helpers */
   int32_t reserved;
   int64_t last_refreshes; /* matrix-core field computations of the last run  */
   int64_t last_flips;     /* accepted flips of the last run (counted on the GPU) */
@@ -189,6 +190,11 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *                    rows from there, 0 reads them from L2
  *   "lds_rows_wpb":  replicas per workgroup of that form (0 = by R: 1 up to
  *                    256 replicas, then up to 4)
+ *   "wave_helpers":  one-wave-per-replica dense contexts of n <= 512: helper
+ *                    waves (1..3) per replica that compute its thresholds one
+ *                    phase of 4 sweeps ahead, 0 = none, -1 (default) = 2 while
+ *                    the replicas number at most twice the CUs, else none;
+ *                    info.waves_per_replica reports 1 + helpers of the last run
  *   "csr_waves":     CSR contexts of degree <= 16: waves sharing one replica's
  *                    sweep (1, 2, 4, 8; 0 = chosen by the replica count)
  *   "update_method": acceptance rule of classical contexts (dense, CSR,

@@ -63,6 +63,35 @@ __device__ __forceinline__ void slot_words(uint32_t seed_lo, uint32_t seed_hi, u
   quad_transpose(w[0], w[1], w[2], w[3], lane);
 }
 
+// One 64-spin slot k of a classical sweep: every lane tests its spin against
+// its threshold th (-inf past n), the lowest accepting lane flips, rows are
+// applied, lanes above it are tested again.  Returns the accepted flips.
+template <typename FT, typename JT, int NS, typename TT>
+__device__ __forceinline__ uint64_t decide_slot(FieldVec<FT, NS>& F, const char* J, int64_t RB,
+                                                int k, int n, int lane, TT th, uint64_t& down,
+                                                int ns) {
+  const int i = 64 * k + lane;
+  uint64_t live = ballot64(i < n);
+  // a lane is never tested again once it has flipped, so every live
+  // lane keeps its sign within the slot: dE = s2 * F
+  const uint64_t sb = ballot64((down >> k) & 1ull);
+  const FT s2 = ((down >> k) & 1ull) ? (FT)2 : (FT)(-2);
+  uint64_t flips = 0;
+  while (true) {
+    const FT fk = F.dyn(k);
+    const FT dE = s2 * fk;
+    const uint64_t cand = ballot64((TT)dE < th) & live;
+    if (cand == 0) break;
+    const int i0 = __builtin_ctzll(cand);
+    live &= lanes_above(i0);
+    const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
+    flips |= 1ull << i0;
+    row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c, ns);
+  }
+  down ^= ((flips >> lane) & 1ull) << k;
+  return (uint64_t)__builtin_popcountll(flips);
+}
+
 // ---------------------------------------------------------------------------
 // Classical: one wave anneals one replica through T sweeps.  `rep` and `slice`
 // are the replica and fourth counter words of its random stream; p0[t] is the
@@ -106,25 +135,7 @@ __device__ __forceinline__ void anneal_replica(const char* J, const FT* h, int8_
         if (k >= ns) break;
         const int i = 64 * k + lane;
         const TT th = (i < n) ? accept_log<TT>(w[jj], rule) * nb : neg_inf<TT>();
-        uint64_t live = ballot64(i < n);
-        // a lane is never tested again once it has flipped, so every live
-        // lane keeps its sign within the slot: dE = s2 * F
-        const uint64_t sb = ballot64((down >> k) & 1ull);
-        const FT s2 = ((down >> k) & 1ull) ? (FT)2 : (FT)(-2);
-        uint64_t flips = 0;
-        while (true) {
-          const FT fk = F.dyn(k);
-          const FT dE = s2 * fk;
-          const uint64_t cand = ballot64((TT)dE < th) & live;
-          if (cand == 0) break;
-          const int i0 = __builtin_ctzll(cand);
-          live &= lanes_above(i0);
-          const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
-          flips |= 1ull << i0;
-          row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c, ns);
-        }
-        down ^= ((flips >> lane) & 1ull) << k;
-        nflip += (uint64_t)__builtin_popcountll(flips);
+        nflip += decide_slot<FT, JT, NS, TT>(F, J, RB, k, n, lane, th, down, ns);
       }
     }
   }

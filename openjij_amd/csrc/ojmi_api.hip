@@ -89,6 +89,11 @@ struct ojmi_ctx {
   int colown = 1;  // shared family: 1 = column-owned kernel when the shape allows
   int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
   int lds_rows_wpb = 0;  // replicas per workgroup of that form (0: by R)
+  // wave family: threshold helper waves per replica (0: none; -1: 2 while the
+  // replicas fit two workgroups per CU and n <= 512, else none)
+  int wave_helpers = -1;
+  int helpers_used = 0;
+  int num_cus = 256;
   void* dJnat = nullptr;   // natural-layout couplings when dJ is packed
   uint32_t* dprog = nullptr;
   int64_t prog_cap = 0;
@@ -234,6 +239,10 @@ int device_lds_budget() {
 int common_init(ojmi_ctx* c) {
   OJMI_HIP(hipGetDevice(&c->device));
   c->lds_budget = device_lds_budget();
+  if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, c->device) !=
+          hipSuccess ||
+      c->num_cus < 1)
+    c->num_cus = 256;
   OJMI_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
   OJMI_HIP(hipEventCreate(&c->e0));
   OJMI_HIP(hipEventCreate(&c->e1));
@@ -679,6 +688,18 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     const size_t lds = wave_lds_bytes((int)c->n, c->NS, jb);
     const bool rows_in_lds = c->lds_rows != 0 && c->NS <= 4 && lds <= (size_t)65536 &&
                              lds <= (size_t)c->lds_budget;
+    c->helpers_used = 0;
+    const int helpers = c->wave_helpers >= 0
+                            ? c->wave_helpers
+                            : ((c->R <= 2 * (int64_t)c->num_cus && c->NS <= 8) ? 2 : 0);
+    if (helpers > 0 && c->NS <= 8) {
+      const int tb = c->ftype == FT_F32 ? 4 : 8;
+      const bool rl = rows_in_lds &&
+                      wave_pipe_lds_bytes((int)c->n, c->NS, jb, tb, true) <= (size_t)c->lds_budget;
+      c->lds_used = (int)wave_pipe_lds_bytes((int)c->n, c->NS, jb, tb, rl);
+      c->helpers_used = helpers;
+      return launch_dense_wave_pipe(c->ftype, c->jtype, a, c->NS, helpers, rl, c->st);
+    }
     if (rows_in_lds) {
       // fewer replicas per workgroup when there are few: more CUs, one LDS copy each
       const int wpb = c->lds_rows_wpb > 0 ? c->lds_rows_wpb
@@ -1321,6 +1342,13 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     c->lds_rows = value ? 1 : 0;
     return OJMI_OK;
   }
+  if (k == "wave_helpers") {
+    if (value < -1 || value > 3) return fail(OJMI_EINVAL, "wave_helpers must be in -1..3");
+    if (value > 0 && (c->family != OJMI_KERNEL_DENSE_WAVE || c->NS > 8))
+      return fail(OJMI_EINVAL, "wave_helpers applies to one-wave dense contexts of n <= 512");
+    c->wave_helpers = (int)value;
+    return OJMI_OK;
+  }
   if (k == "lds_rows_wpb") {
     if (value < 0 || value > 4) return fail(OJMI_EINVAL, "lds_rows_wpb must be 0..4");
     c->lds_rows_wpb = (int)value;
@@ -1705,7 +1733,9 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->lds_budget = c->lds_budget;
   info->num_problems = (c->kind == 2) ? c->K : 1;
   info->waves_per_replica =
-      (c->family == OJMI_KERNEL_CSR_LEVEL) ? csr_waves_per_replica(c->R, c->D, c->csr_waves, (int)c->n, c->nlev, c->ftype) : 1;
+      (c->family == OJMI_KERNEL_CSR_LEVEL)
+          ? csr_waves_per_replica(c->R, c->D, c->csr_waves, (int)c->n, c->nlev, c->ftype)
+          : 1 + c->helpers_used;
   info->reserved = 0;
   info->last_refreshes = c->last_refreshes;
   info->last_flips = c->last_flips;

@@ -110,6 +110,11 @@ hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, h
 size_t wave_lds_bytes(int n, int NS, int jbytes);
 hipError_t launch_dense_wave_lds(int ftype, int jtype, const DenseArgs& a, int NS, int wpb,
                                  hipStream_t st);
+// The wave anneal of one replica per workgroup with H = 1..3 helper waves
+// making its thresholds one phase ahead (NS <= 8); rows from LDS or L2.
+size_t wave_pipe_lds_bytes(int n, int NS, int jbytes, int tbytes, bool rows_lds);
+hipError_t launch_dense_wave_pipe(int ftype, int jtype, const DenseArgs& a, int NS, int H,
+                                  bool rows_lds, hipStream_t st);
 hipError_t launch_batch_wave(int ftype, int jtype, const BatchArgs& b, int NS, hipStream_t st);
 // Classical dense systems too large for register-resident fields (k_lds.hip):
 // natural coupling rows of Npad = 64 * ceil(n / 64), fields in LDS.

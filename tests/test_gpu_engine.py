@@ -143,6 +143,7 @@ def test_dense_wave_rows_in_lds_bitexact(n, prec, R, wpb):
     out = {}
     for lds in (1, 0):
         with capi.Context.dense(E, R, 1, 0.0, PREC[prec]) as ctx:
+            ctx.set_option("wave_helpers", 0)  # the form without threshold helpers
             ctx.set_option("lds_rows", lds)
             ctx.set_option("lds_rows_wpb", wpb)
             ctx.set_spins(init)
@@ -157,6 +158,36 @@ def test_dense_wave_rows_in_lds_bitexact(n, prec, R, wpb):
                             orc.expand_classical(beta, mcs))
     np.testing.assert_array_equal(out[1], want)
     np.testing.assert_array_equal(out[0], want)
+
+
+@pytest.mark.parametrize("n,prec,R,L", [(256, "int32", 64, 37), (200, "int32", 3, 9),
+                                        (500, "int32", 5, 6), (64, "fp64", 9, 13),
+                                        (130, "fp32", 7, 5), (40, "fp64", 2, 2), (512, "fp32", 3, 4)])
+@pytest.mark.parametrize("helpers", [1, 2, 3])
+def test_dense_wave_threshold_helpers_bitexact(n, prec, R, L, helpers):
+    """One replica per workgroup, thresholds made one phase ahead by helper
+    waves (phases of 4 sweeps, schedules of any length): the replay's
+    trajectory, and a run split in two carries its fields as before."""
+    E = H.sk_pm1(n, 0x4E1 + n) if prec == "int32" else H.sk_gauss(n, 0x4E1 + n, field=True)
+    beta, mcs = sched(L, 0.1, 6.0)
+    init = H.random_pm1(R, n, 23)
+    want = orc.replay_dense(prec, H.symmetric_ext(E), init, 0x4E1, 0,
+                            orc.expand_classical(beta, mcs))
+    with capi.Context.dense(E, R, 1, 0.0, PREC[prec]) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_DENSE_WAVE
+        ctx.set_option("wave_helpers", helpers)
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 0x4E1, 0)
+        info = ctx.info()
+        assert info["waves_per_replica"] == 1 + helpers
+        np.testing.assert_array_equal(ctx.get_spins(), want)
+        ctx.set_spins(init)
+        h = L // 2
+        ctx.run(beta[:h], mcs[:h], 0x4E1, 0)
+        ctx.run(beta[h:], mcs[h:], 0x4E1, h)
+        np.testing.assert_array_equal(ctx.get_spins(), want)
+        np.testing.assert_array_equal(ctx.energies(), orc.energy_exact(E, want)) \
+            if prec == "int32" else None
 
 
 # --- CSR --------------------------------------------------------------------
