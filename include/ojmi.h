@@ -59,6 +59,7 @@ extern "C" {
 #define OJMI_KERNEL_DENSE_LDS 7    /* N > 4096: one workgroup per replica, fields in LDS */
 #define OJMI_KERNEL_BATCH_PERSIST 8 /* batch of problems, one persistent launch, LPT queues */
 #define OJMI_KERNEL_BATCH_FUSED 9   /* batch of problems, one launch, largest problem first */
+#define OJMI_KERNEL_DENSE_MFMA 10   /* 8 replicas per workgroup, row updates on the matrix cores */
 
 typedef struct ojmi_ctx ojmi_ctx;
 

@@ -87,6 +87,10 @@ struct ojmi_ctx {
   int lds_budget = 65536;
   int ring_rows_opt = 0, round_rows_opt = 0, ring_rows_used = 0, lds_used = 0;
   int colown = 1;  // shared family: 1 = column-owned kernel when the shape allows
+  // dense classical FP32 / INT32 (int8 couplings): 1 = row updates on the
+  // matrix cores (k_dense_mfma), when the shape allows
+  int mfma_sweep = 0;
+  void* dJpk = nullptr;  // packed MFMA operands of the couplings (k_dense_mfma)
   int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
   int lds_rows_wpb = 0;  // replicas per workgroup of that form (0: by R)
   // wave family: threshold helper waves per replica (0: none; -1: 2 while the
@@ -172,7 +176,7 @@ struct ojmi_ctx {
                     dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
                     d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
                     dJhi, dJlo, dS8, dSbf, dpart, dstats, d_q_items, d_q_begin, d_q_head,
-                    d_fused};
+                    d_fused, dJpk};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     for (auto& cl : bclass)
@@ -581,6 +585,30 @@ bool carries_fields(const ojmi_ctx* c) {
   return c->kind == 0 && c->M == 1 && (c->ftype == FT_F32 || c->ftype == FT_F64);
 }
 
+// The matrix-core sweep applies: classical dense FP32 (bf16 hi/lo operands
+// present) or INT32 with int8 couplings, 4..64 slots, field buffer offsets
+// within 2 GiB (32-bit buffer offsets).
+bool mfma_sweep_shape(const ojmi_ctx* c) {
+  return c->kind == 0 && c->M == 1 &&
+         (c->family == OJMI_KERNEL_DENSE_WAVE || c->family == OJMI_KERNEL_DENSE_SHARED) &&
+         mfma_sweep_supported(c->NS) &&
+         ((c->ftype == FT_F32 && c->dJhi) || (c->ftype == FT_I32 && c->jtype == JT_I8 && c->dJnat)) &&
+         c->R * c->Npad * 4 < ((int64_t)1 << 31);
+}
+bool uses_mfma_sweep(const ojmi_ctx* c) { return c->mfma_sweep && mfma_sweep_shape(c); }
+
+// Packed operands of the matrix-core sweep, built once on the device.
+int ensure_mfma_packed(ojmi_ctx* c) {
+  if (c->dJpk) return OJMI_OK;
+  char* p;
+  int rc;
+  if ((rc = dalloc(&p, mfma_packed_bytes(c->ftype, c->NS)))) return rc;
+  c->dJpk = p;
+  const void* src = c->ftype == FT_F32 ? (const void*)c->dJhi : (const void*)c->dJnat;
+  OJMI_HIP(launch_pack_mfma(c->ftype, src, c->dJlo, c->NS, c->dJpk, c->st));
+  return OJMI_OK;
+}
+
 // Zero-padded spins for the matrix-core kernels: int8 [R*M][Npad] (fields of
 // INT32 contexts, energies of every context) and bf16 [R][Npad] (FP32 fields).
 int ensure_mfma_spins(ojmi_ctx* c) {
@@ -677,6 +705,36 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
   if (c->family == OJMI_KERNEL_SQA_WAVE) {
     a.F_in = a.F_out = nullptr;
     return launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
+  }
+  if (uses_mfma_sweep(c)) {
+    SharedArgs sa{};
+    sa.J = c->ftype == FT_F32 ? c->dJ : c->dJnat;
+    sa.h = c->dh;
+    sa.spins = c->dspins;
+    sa.n = (int)c->n;
+    sa.R = (int)c->R;
+    sa.T = (int)T;
+    sa.seed = seed;
+    sa.t0 = sweep_offset + (uint64_t)t_begin;
+    sa.rule = c->rule;
+    sa.rep_offset = 0;
+    sa.p0_d = c->dp0_d + t_begin;
+    sa.p0_f = c->dp0_f + t_begin;
+    sa.F_in = F_in;
+    sa.F_out = F_out;
+    sa.stats = c->dstats;
+    const int64_t nwg = (c->R + 7) / 8;
+    sa.xcd_window = (nwg >= 16) ? c->xcd_window : 0;
+    sa.xcd_spin = 4000;
+    if (sa.xcd_window > 0) {
+      if (c->prog_cap < nwg) return hipErrorInvalidValue;
+      hipError_t e = hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st);
+      if (e != hipSuccess) return e;
+      sa.progress = c->dprog;
+    }
+    c->lds_used = (int)mfma_sweep_lds_bytes(c->NS, 4, c->ftype == FT_F32 ? 4 : 1);
+    c->ring_rows_used = 0;
+    return launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->st);
   }
   if (c->family == OJMI_KERNEL_DENSE_LDS) {
     if (c->jtype == JT_I8) a.J = c->dJnat;
@@ -1337,6 +1395,15 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     c->colown = (int)value;
     return OJMI_OK;
   }
+  if (k == "mfma_sweep") {
+    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "mfma_sweep must be 0 or 1");
+    if (value == 1 && !mfma_sweep_shape(c))
+      return fail(OJMI_ENOTSUP,
+                  "mfma_sweep applies to classical dense FP32 or int8-coupling INT32 contexts "
+                  "of 193..4096 spins");
+    c->mfma_sweep = (int)value;
+    return OJMI_OK;
+  }
   if (k == "lds_rows") {
     if (value != 0 && value != 1) return fail(OJMI_EINVAL, "lds_rows must be 0 or 1");
     c->lds_rows = value ? 1 : 0;
@@ -1485,7 +1552,9 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
   if (c->family != OJMI_KERNEL_CSR_LEVEL && c->family != OJMI_KERNEL_SQA_WAVE &&
       (rc = ensure_fields(c)))
     return rc;
-  if ((c->family == OJMI_KERNEL_DENSE_SHARED) && (rc = ensure_progress(c))) return rc;
+  const bool msweep = uses_mfma_sweep(c);
+  if ((c->family == OJMI_KERNEL_DENSE_SHARED || msweep) && (rc = ensure_progress(c))) return rc;
+  if (msweep && (rc = ensure_mfma_packed(c))) return rc;
   c->energies_valid = false;
   if ((rc = begin_stats(c))) return rc;
 
@@ -1496,14 +1565,16 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
   } else {
     // Fields: from the matrix cores (exact int8 for INT32 couplings; bf16
     // hi/lo for FP32 when a refresh period is set, refreshed every
-    // refresh_every sweeps), else built by the sweep kernel itself in sweep
-    // order (the FP32 replay's order).
+    // refresh_every sweeps, or when the matrix-core sweep starts without
+    // carried fields), else built by the sweep kernel itself in sweep order
+    // (the FP32 replay's order).
     const bool mfma_i8 = c->ftype == FT_I32 && c->dJnat && c->M == 1;
-    const bool mfma_bf = c->ftype == FT_F32 && c->dJhi && c->refresh_every > 0;
-    const int64_t chunk = mfma_bf ? c->refresh_every : T;
+    const bool refresh_bf = c->ftype == FT_F32 && c->dJhi && c->refresh_every > 0;
+    const int64_t chunk = refresh_bf ? c->refresh_every : T;
     for (int64_t t = 0; t < T && e == hipSuccess; t += chunk) {
       const int64_t Tc = std::min<int64_t>(chunk, T - t);
       const void* F_in = (t == 0 && c->fields_valid && carries_fields(c)) ? c->dF : nullptr;
+      const bool mfma_bf = refresh_bf || (msweep && c->ftype == FT_F32 && !F_in);
       if (mfma_i8 || mfma_bf) {
         if (mfma_bf) {
           if ((size_t)(2 * c->last_refreshes + 2) > c->ev_ref.size()) {
@@ -1721,13 +1792,14 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
   info->kernel = c->family;
   if (c->family == OJMI_KERNEL_DENSE_SHARED && c->colown && colown_supported((int)c->NS, c->group))
     info->kernel = OJMI_KERNEL_DENSE_COLOWN;
+  if (uses_mfma_sweep(c)) info->kernel = OJMI_KERNEL_DENSE_MFMA;
   info->refresh_every = c->refresh_every;
   info->coupling_bytes = (c->jtype == JT_F64) ? 8 : (c->jtype == JT_I8 ? 1 : 4);
   info->last_sweeps = c->last_T;
   info->last_attempts = c->last_attempts;
   info->last_kernel_ms = c->last_ms;
   info->last_refresh_ms = c->last_refresh_ms;
-  info->group = c->group;
+  info->group = uses_mfma_sweep(c) ? 8 : c->group;
   info->ring_rows = c->ring_rows_used;
   info->lds_bytes = c->lds_used;
   info->lds_budget = c->lds_budget;

@@ -104,6 +104,18 @@ bool colown_supported(int NS, int G);
 hipError_t launch_dense_colown(int ftype, int jtype, const SharedArgs& a, int NS, int G,
                                hipStream_t st);
 
+// Dense sweep with the row updates on the matrix cores (k_mfma_sweep.hip):
+// 8 replicas per workgroup, FP32 fields with bf16 hi/lo couplings or INT32
+// fields with int8 couplings; a.J natural couplings (diagonal tiles), Jpk the
+// packed operands written by launch_pack_mfma, a.F_in required.
+size_t mfma_sweep_lds_bytes(int NS, int fbytes, int jbytes);
+bool mfma_sweep_supported(int NS);
+size_t mfma_packed_bytes(int ftype, int NS);
+// FP32: Jhi, Jlo natural bf16 couplings; INT32: Jhi natural int8, Jlo unused.
+hipError_t launch_pack_mfma(int ftype, const void* Jhi, const void* Jlo, int NS, void* out,
+                            hipStream_t st);
+hipError_t launch_dense_mfma(int ftype, const SharedArgs& a, const void* Jpk, int NS,
+                             hipStream_t st);
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
 // The wave kernel with the coupling rows staged in LDS (NS <= 4): bytes of
 // LDS per workgroup, and the launch with wpb waves (replicas) per workgroup.
