@@ -325,7 +325,7 @@ def profiled(kernel_name, workload_name):
 
 KERNEL_NAMES = {1: "k_dense_wave", 2: "k_sqa_wave", 3: "k_csr_level", 4: "k_dense_shared",
                 5: "k_dense_colown", 6: "k_batch_wave", 7: "k_dense_lds", 8: "k_batch_persist",
-                9: "k_batch_wave"}
+                9: "k_batch_wave", 10: "k_dense_mfma"}
 
 
 def kernel_name(info):
@@ -468,6 +468,7 @@ def run_config(cfg, steps, warmup, with_cpu, with_plain, opts, sweeps=None, repl
                    "problems": len(W["ns"]) if W["kind"] == "batch" else 1,
                    "trotter": W["M"], "sweeps": int(W["mcs"].sum()), "path": W["fast"],
                    "kernel": kernel_name(info), "refresh_every": info["refresh_every"],
+                   "mfma_segments": info["mfma_segments"],
                    "group": info["group"], "lds_bytes": info["lds_bytes"],
                    "anneal_ms": float(kms.mean()), "refresh_ms": float(rms.mean()),
                    "sweep_kernel_ms": sweep_ms},

@@ -82,7 +82,7 @@ typedef struct ojmi_info {
   int64_t num_problems;   /* problems of a batch context (1 otherwise)       */
   int32_t waves_per_replica; /* CSR: waves sharing one replica's sweep;     */
                              /* dense one-wave form: 1 + threshold helpers */
-  int32_t reserved;
+  int32_t mfma_segments;  /* sweep segments of the last run on the matrix-core sweep */
   int64_t last_refreshes; /* matrix-core field computations of the last run  */
   int64_t last_flips;     /* accepted flips of the last run (counted on the GPU) */
   int64_t last_rows;      /* coupling rows the last run streamed: each row read

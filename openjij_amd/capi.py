@@ -30,7 +30,7 @@ class Info(C.Structure):
         ("last_attempts", C.c_int64), ("last_kernel_ms", C.c_double),
         ("last_refresh_ms", C.c_double), ("group", C.c_int32), ("ring_rows", C.c_int32),
         ("lds_bytes", C.c_int32), ("lds_budget", C.c_int32), ("num_problems", C.c_int64),
-        ("waves_per_replica", C.c_int32), ("reserved", C.c_int32),
+        ("waves_per_replica", C.c_int32), ("mfma_segments", C.c_int32),
         ("last_refreshes", C.c_int64), ("last_flips", C.c_int64), ("last_rows", C.c_int64),
     ]
 

@@ -183,6 +183,7 @@ void k_dense_mfma(SharedArgs a, const char* __restrict__ Jpk) {
   constexpr int TL = 16 / MW;         // tile loads per wave
   constexpr int TILE_BYTES = 64 * 64 * (int)sizeof(JT);
   static_assert(NS % MW == 0 && NI % GS == 0 && TP >= 2, "shape");
+  if (a.sel && *a.sel != a.sel_id) return;  // another kernel runs this segment
 
   extern __shared__ __align__(16) char smem[];
   char* sD = smem;                                             // diagonal tile
@@ -413,6 +414,29 @@ void k_dense_mfma(SharedArgs a, const char* __restrict__ Jpk) {
       __builtin_amdgcn_sched_barrier(0);
     }
   }
+}
+
+__global__ void k_sweep_select(const unsigned long long* __restrict__ stats, int32_t* sel,
+                               uint64_t attempts, int permille, int init) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long* snap = reinterpret_cast<unsigned long long*>(sel + 2);
+  const unsigned long long now = stats[0];
+  if (init) {
+    sel[0] = 1;
+    sel[1] = 0;
+  } else {
+    sel[1] += sel[0];  // the segment just run
+    const unsigned long long f = now - *snap;
+    sel[0] = (f * 1000ull >= (unsigned long long)permille * attempts) ? 1 : 0;
+  }
+  *snap = now;
+}
+
+hipError_t launch_sweep_select(const unsigned long long* stats, int32_t* sel, uint64_t attempts,
+                               int permille, int init, hipStream_t st) {
+  hipLaunchKernelGGL(k_sweep_select, dim3(1), dim3(64), 0, st, stats, sel, attempts, permille,
+                     init);
+  return hipGetLastError();
 }
 
 size_t mfma_sweep_lds_bytes(int NS, int fbytes, int jbytes) {

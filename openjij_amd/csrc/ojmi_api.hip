@@ -89,8 +89,12 @@ struct ojmi_ctx {
   int colown = 1;  // shared family: 1 = column-owned kernel when the shape allows
   // dense classical FP32 / INT32 (int8 couplings): 1 = row updates on the
   // matrix cores (k_dense_mfma), when the shape allows
+  // (2: per 16-sweep segment, by the acceptance rate the GPU measured in the
+  // previous one, against the column-owned kernel)
   int mfma_sweep = 0;
+  int mfma_accept = 110;  // per mille: the matrix-core sweep at or above this rate
   void* dJpk = nullptr;  // packed MFMA operands of the couplings (k_dense_mfma)
+  int32_t* dsel = nullptr;  // segment kernel choice and flip snapshot (mode 2)
   int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
   int lds_rows_wpb = 0;  // replicas per workgroup of that form (0: by R)
   // wave family: threshold helper waves per replica (0: none; -1: 2 while the
@@ -143,6 +147,7 @@ struct ojmi_ctx {
   int device = 0;  // current device when the context was created
 
   int64_t last_T = 0, last_attempts = 0;
+  int64_t last_mfma_segments = 0;  // segments of the last run on the matrix-core sweep
   double last_ms = 0, last_refresh_ms = 0;
 
   // batch of independent problems (kind 2): per-problem sizes and offsets
@@ -176,7 +181,7 @@ struct ojmi_ctx {
                     dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
                     d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
                     dJhi, dJlo, dS8, dSbf, dpart, dstats, d_q_items, d_q_begin, d_q_head,
-                    d_fused, dJpk};
+                    d_fused, dJpk, dsel};
     for (void* b : bufs)
       if (b) (void)hipFree(b);
     for (auto& cl : bclass)
@@ -596,6 +601,13 @@ bool mfma_sweep_shape(const ojmi_ctx* c) {
          c->R * c->Npad * 4 < ((int64_t)1 << 31);
 }
 bool uses_mfma_sweep(const ojmi_ctx* c) { return c->mfma_sweep && mfma_sweep_shape(c); }
+// Mode 2 alternates with the column-owned kernel, which needs the
+// workgroup-shared family in its column-owned form.
+bool colown_form(const ojmi_ctx* c) {
+  return c->family == OJMI_KERNEL_DENSE_SHARED && c->colown &&
+         colown_supported((int)c->NS, c->group) && c->rule == RULE_METROPOLIS;
+}
+bool hybrid_sweep(const ojmi_ctx* c) { return c->mfma_sweep == 2 && uses_mfma_sweep(c); }
 
 // Packed operands of the matrix-core sweep, built once on the device.
 int ensure_mfma_packed(ojmi_ctx* c) {
@@ -656,6 +668,10 @@ hipError_t mfma_fields(ojmi_ctx* c) {
 // [t_begin, t_begin + T): global sweep indices sweep_offset + t_begin + t.
 // F_in: initial fields ([R][Npad], the field type) or null (the kernel builds
 // them in sweep order).  FP32 contexts write their final fields to dF.
+hipError_t launch_shared_family(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
+                                uint64_t sweep_offset, const void* F_in, void* F_out,
+                                const int32_t* sel, int sel_id);
+
 hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
                          uint64_t sweep_offset, const void* F_in) {
   if (c->family == OJMI_KERNEL_CSR_LEVEL) {
@@ -734,7 +750,23 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     }
     c->lds_used = (int)mfma_sweep_lds_bytes(c->NS, 4, c->ftype == FT_F32 ? 4 : 1);
     c->ring_rows_used = 0;
-    return launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->st);
+    if (!hybrid_sweep(c)) return launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->st);
+    // one segment: the kernel the selector chose runs, the other returns at once
+    hipError_t e;
+    if (t_begin == 0 &&
+        (e = launch_sweep_select(c->dstats, c->dsel, 0, c->mfma_accept, 1, c->st)) != hipSuccess)
+      return e;
+    if ((e = launch_shared_family(c, t_begin, T, seed, sweep_offset, F_in, F_out, c->dsel, 0)) !=
+        hipSuccess)
+      return e;
+    sa.sel = c->dsel;
+    sa.sel_id = 1;
+    if (sa.xcd_window > 0 &&
+        (e = hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st)) != hipSuccess)
+      return e;
+    if ((e = launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->st)) != hipSuccess) return e;
+    const uint64_t attempts = (uint64_t)c->R * (uint64_t)c->n * (uint64_t)T;
+    return launch_sweep_select(c->dstats, c->dsel, attempts, c->mfma_accept, 0, c->st);
   }
   if (c->family == OJMI_KERNEL_DENSE_LDS) {
     if (c->jtype == JT_I8) a.J = c->dJnat;
@@ -769,8 +801,15 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     return launch_dense_wave(c->ftype, c->jtype, a, c->NS, c->st);
   }
 
-  // workgroup-shared families
+  return launch_shared_family(c, t_begin, T, seed, sweep_offset, F_in, F_out, nullptr, 0);
+}
+
+hipError_t launch_shared_family(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
+                                uint64_t sweep_offset, const void* F_in, void* F_out,
+                                const int32_t* sel, int sel_id) {
   SharedArgs sa{};
+  sa.sel = sel;
+  sa.sel_id = sel_id;
   sa.J = c->dJnat ? c->dJnat : c->dJ;
   sa.h = c->dh;
   sa.spins = c->dspins;
@@ -898,6 +937,20 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
       }
       break;
     }
+  }
+  // Large FP32 systems with replicas enough to fill the device: per 16-sweep
+  // segment the matrix-core sweep while at least 11% of the flip attempts are
+  // accepted, the column-owned kernel below (measured on dense SK N = 4096,
+  // 2048 replicas: 1.05 -> 0.66 ms per sweep for the matrix-core sweep from
+  // acceptance 0.91 to 0.005, 2.0 -> 0.45 ms for the column-owned one,
+  // crossing at 0.11)
+  if (c->ftype == FT_F32 && c->M == 1 && c->NS >= 32 && c->R >= 2048 && colown_form(c) &&
+      mfma_sweep_shape(c)) {
+    if ((rc = dalloc(&c->dsel, 4))) {
+      delete c;
+      return rc;
+    }
+    c->mfma_sweep = 2;
   }
   *out = c;
   return OJMI_OK;
@@ -1396,12 +1449,27 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     return OJMI_OK;
   }
   if (k == "mfma_sweep") {
-    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "mfma_sweep must be 0 or 1");
-    if (value == 1 && !mfma_sweep_shape(c))
+    // 0: off; 1: the matrix-core sweep; 2: per 16-sweep segment, the
+    // matrix-core sweep while the acceptance rate is at least mfma_accept per
+    // mille, else the column-owned kernel
+    if (value < 0 || value > 2) return fail(OJMI_EINVAL, "mfma_sweep must be 0, 1 or 2");
+    if (value >= 1 && !mfma_sweep_shape(c))
       return fail(OJMI_ENOTSUP,
                   "mfma_sweep applies to classical dense FP32 or int8-coupling INT32 contexts "
-                  "of 193..4096 spins");
+                  "of 513..4096 spins");
+    if (value == 2 && !colown_form(c))
+      return fail(OJMI_ENOTSUP,
+                  "mfma_sweep 2 alternates with the column-owned kernel (Metropolis rule)");
+    if (value == 2 && !c->dsel) {
+      int rc = dalloc(&c->dsel, 4);
+      if (rc) return rc;
+    }
     c->mfma_sweep = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "mfma_accept") {
+    if (value < 0 || value > 1000) return fail(OJMI_EINVAL, "mfma_accept must be 0..1000");
+    c->mfma_accept = (int)value;
     return OJMI_OK;
   }
   if (k == "lds_rows") {
@@ -1518,6 +1586,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
   c->last_refresh_ms = 0;
   c->last_refreshes = 0;
   c->last_flips = c->last_rows = 0;
+  c->last_mfma_segments = 0;
   if (T == 0) return OJMI_OK;
   int rc;
   if ((rc = ensure_sched(c, T))) return rc;
@@ -1570,10 +1639,12 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     // (the FP32 replay's order).
     const bool mfma_i8 = c->ftype == FT_I32 && c->dJnat && c->M == 1;
     const bool refresh_bf = c->ftype == FT_F32 && c->dJhi && c->refresh_every > 0;
-    const int64_t chunk = refresh_bf ? c->refresh_every : T;
+    const int64_t chunk = refresh_bf ? c->refresh_every : (hybrid_sweep(c) ? 16 : T);
     for (int64_t t = 0; t < T && e == hipSuccess; t += chunk) {
       const int64_t Tc = std::min<int64_t>(chunk, T - t);
-      const void* F_in = (t == 0 && c->fields_valid && carries_fields(c)) ? c->dF : nullptr;
+      // carried fields: from the previous run, or the previous segment
+      const void* F_in =
+          ((t == 0 && c->fields_valid) || t > 0) && carries_fields(c) ? c->dF : nullptr;
       const bool mfma_bf = refresh_bf || (msweep && c->ftype == FT_F32 && !F_in);
       if (mfma_i8 || mfma_bf) {
         if (mfma_bf) {
@@ -1593,6 +1664,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
         F_in = c->dF;
       }
       e = launch_sweeps(c, t, Tc, seed, sweep_offset, F_in);
+      if (msweep && !hybrid_sweep(c)) ++c->last_mfma_segments;
     }
     c->fields_valid = carries_fields(c);
   }
@@ -1606,6 +1678,11 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
     float rm = 0;
     OJMI_HIP(hipEventElapsedTime(&rm, c->ev_ref[2 * q], c->ev_ref[2 * q + 1]));
     c->last_refresh_ms += rm;
+  }
+  if (hybrid_sweep(c)) {
+    int32_t sel[2] = {0, 0};
+    OJMI_HIP(hipMemcpy(sel, c->dsel, sizeof(sel), hipMemcpyDeviceToHost));
+    c->last_mfma_segments = sel[1];
   }
   return end_stats(c);
 }
@@ -1808,7 +1885,7 @@ int ojmi_get_info(ojmi_ctx* c, ojmi_info* info) {
       (c->family == OJMI_KERNEL_CSR_LEVEL)
           ? csr_waves_per_replica(c->R, c->D, c->csr_waves, (int)c->n, c->nlev, c->ftype)
           : 1 + c->helpers_used;
-  info->reserved = 0;
+  info->mfma_segments = (int32_t)c->last_mfma_segments;
   info->last_refreshes = c->last_refreshes;
   info->last_flips = c->last_flips;
   info->last_rows = c->last_rows;

@@ -182,6 +182,7 @@ void k_dense_colown(SharedArgs a) {
   constexpr int TILE_BYTES = 64 * 64 * (int)sizeof(JT);
   constexpr int PF = (CS >= 16) ? 2 : ((CS >= 4) ? 32 / CS : 8);  // rows per prefetch group
 
+  if (a.sel && *a.sel != a.sel_id) return;  // another kernel runs this segment
   extern __shared__ __align__(16) char smem[];
   char* sD = smem;                                                   // 1 tile
   FT* sF = reinterpret_cast<FT*>(smem + TILE_BYTES);                 // [G][64]

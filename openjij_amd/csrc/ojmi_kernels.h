@@ -90,6 +90,10 @@ struct SharedArgs {
   uint32_t* progress;   // [workgroups] blocks completed
   int rule;
   unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
+  // optional kernel selector (column-owned and matrix-core sweeps): the kernel
+  // returns at once unless *sel == sel_id
+  const int32_t* sel = nullptr;
+  int sel_id = 0;
 };
 
 size_t shared_lds_bytes(int NS, int G, int ring_rows, int fbytes);
@@ -116,6 +120,13 @@ hipError_t launch_pack_mfma(int ftype, const void* Jhi, const void* Jlo, int NS,
                             hipStream_t st);
 hipError_t launch_dense_mfma(int ftype, const SharedArgs& a, const void* Jpk, int NS,
                              hipStream_t st);
+// Kernel choice of the next sweep segment from the accepted flips of the last
+// one: sel = 1 (matrix-core sweep) when flips * 1000 >= permille * attempts,
+// else 0; init: start a run (sel = 1, flip snapshot taken).  sel[0] is the
+// choice, sel[1] counts the segments run on the matrix-core sweep, sel[2..3]
+// the snapshot.
+hipError_t launch_sweep_select(const unsigned long long* stats, int32_t* sel, uint64_t attempts,
+                               int permille, int init, hipStream_t st);
 hipError_t launch_dense_wave(int ftype, int jtype, const DenseArgs& a, int NS, hipStream_t st);
 // The wave kernel with the coupling rows staged in LDS (NS <= 4): bytes of
 // LDS per workgroup, and the launch with wpb waves (replicas) per workgroup.

@@ -151,3 +151,67 @@ def test_option_rejected_where_unsupported():
     with capi.Context.dense(H.sk_gauss(1000, 6), 8, 1, 0.0, capi.PREC_FP64) as ctx:
         with pytest.raises(capi.OjmiError):
             ctx.set_option("mfma_sweep", 1)
+
+
+def test_int32_hybrid_segments_bitexact():
+    """mfma_sweep = 2: each 16-sweep segment runs the matrix-core sweep or the
+    column-owned kernel by the acceptance rate the GPU counted in the segment
+    before.  Both are exact for INT32 fields, so the trajectory stays the
+    oracle replay's whichever ran."""
+    n, R = 1024, 64
+    E = H.int_dense(n, 0x3E00)
+    L = 64
+    beta, mcs = H.geometric_betas(0.05, 4.0, L), np.ones(L, dtype=np.int64)
+    init = H.random_pm1(R, n, 31)
+    with capi.Context.dense(E, R, 1, 0.0, capi.PREC_INT32) as ctx:
+        ctx.set_option("group", 4)
+        ctx.set_option("mfma_sweep", 2)
+        ctx.set_option("mfma_accept", 300)  # switches within this schedule
+        assert ctx.info()["kernel"] == capi.KERNEL_DENSE_MFMA
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 0xAB, 3)
+        got = ctx.get_spins()
+        segs = ctx.info()["mfma_segments"]
+    print(f"segments on the matrix-core sweep: {segs} of 4")
+    assert 0 < segs < 4  # both kernels ran
+    want = orc.replay_dense("int32", H.symmetric_ext(E), init, 0xAB, 3,
+                            orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(got, want)
+
+
+def test_fp32_default_large_context_is_hybrid_and_replays():
+    """Dense FP32 with 2048 replicas and 4096 spins (the SK N = 4096 shape):
+    the default is mfma_sweep = 2 with a 16-sweep field refresh.  A hot
+    segment (matrix-core sweep) then two cold ones (column-owned kernel)
+    against the FP32 replay of 16 of the replicas, each segment replayed from
+    the fields the GPU's refresh gives for its starting spins."""
+    from tests.test_gpu_mfma import chained_replay
+
+    n, R, K = 4096, 2048, 16
+    E = H.sk_gauss(n, 0xE005)
+    beta = np.concatenate([np.full(K, 0.2), np.full(2 * K, 6.0)])
+    mcs = np.ones(len(beta), dtype=np.int64)
+    init = H.random_pm1(R, n, 41)
+    with capi.Context.dense(E, R, 1, 0.0, capi.PREC_FP32) as ctx:
+        info = ctx.info()
+        assert info["kernel"] == capi.KERNEL_DENSE_MFMA and info["group"] == 8
+        ctx.set_option("refresh_every", K)
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 0x5151, 0)
+        got = ctx.get_spins()[:16]
+        F = ctx.get_fields()[:16]
+        en = ctx.energies()
+        segs = ctx.info()["mfma_segments"]
+    print(f"segments on the matrix-core sweep: {segs} of 3")
+    assert 0 < segs < 3  # hot segments on the matrix cores, the last cold one not
+    with capi.Context.dense(E, 16, 1, 0.0, capi.PREC_FP32) as refresher:
+        want = chained_replay(E, init[:16], beta, mcs, K, 0x5151, 0, refresher)
+    same = np.all(got == want, axis=1)
+    print(f"replicas identical to the segment-wise FP32 replay: {same.sum()}/16")
+    assert same.mean() >= 0.75
+    err = np.abs(F - exact_fields(E, got))
+    scale = np.abs(H.symmetric_ext(E)[:n]).sum(axis=1)
+    print(f"max |F - F_exact| = {err.max():.3e}")
+    assert np.all(err <= 2e-6 * scale[None, :])
+    np.testing.assert_allclose(en[:16], orc.energy_exact(E, got),
+                               rtol=1e-12, atol=1e-9)

@@ -1,9 +1,11 @@
 """The exact kernel instances bench.py times, under bit-exact parity.
 
-* Dense SK N=4096, FP32: the launch shape a 2048-replica context picks
-  (the autotuner's table; 4 replicas per workgroup with column-owned fields,
-  16 slots per wave chunk, as tuned on MI355X), run for a few sweeps of 8
-  replicas against the FP32 replay.
+* Dense SK N=4096, FP32: a 2048-replica context alternates, per 16-sweep
+  segment, between the matrix-core sweep (8 replicas per workgroup; its
+  instance at this shape is pinned in test_gpu_mfma_sweep.py) and the
+  column-owned kernel of the autotuner's shape (4 replicas per workgroup,
+  16 slots per wave chunk, as tuned on MI355X); the latter runs here for a
+  few sweeps of 8 replicas against the FP32 replay.
 * 4096 uneven QUBOs, 16 reads each: 128 problems drawn from the same size
   distribution, QUBO -> Ising with the exact x4 scaling, so that the
   workgroup order maps whole rounds of problems onto XCDs (non-identity
@@ -28,13 +30,15 @@ def test_config_e_kernel_shape_bitexact():
     init = H.random_pm1(R, 4096, 0xE1)
     with capi.Context.dense(E, 2048, 1, 0.0, capi.PREC_FP32) as big:
         shape = big.info()  # the launch shape config E's context runs with
-    assert shape["kernel"] in (capi.KERNEL_DENSE_COLOWN, capi.KERNEL_DENSE_SHARED,
-                               capi.KERNEL_DENSE_WAVE)
+        assert (shape["kernel"], shape["group"]) == (capi.KERNEL_DENSE_MFMA, 8)
+        big.set_option("mfma_sweep", 0)
+        colown = big.info()  # the column-owned kernel it alternates with
+    assert (colown["kernel"], colown["group"]) == (capi.KERNEL_DENSE_COLOWN, 4)
     with capi.Context.dense(E, R, 1, 0.0, capi.PREC_FP32) as ctx:
-        ctx.set_option("group", shape["group"])
-        ctx.set_option("colown", 1 if shape["kernel"] == capi.KERNEL_DENSE_COLOWN else 0)
+        ctx.set_option("group", colown["group"])
+        ctx.set_option("colown", 1)
         info = ctx.info()
-        assert (info["kernel"], info["group"]) == (shape["kernel"], shape["group"])
+        assert (info["kernel"], info["group"]) == (colown["kernel"], colown["group"])
         ctx.set_spins(init)
         ctx.run(beta, mcs, 1000, 0)
         got = ctx.get_spins()
