@@ -416,6 +416,332 @@ void k_dense_mfma(SharedArgs a, const char* __restrict__ Jpk) {
   }
 }
 
+// Split-role form of the same sweep: waves 0..3 hold the fields (tile pairs
+// [s NS/2, (s+1) NS/2) of wave s) and stream each block's update, waves 4..7
+// decide (wave 4 + d for replicas d and d + 4, the two chains interleaved).
+// Per block k, between two workgroup barriers B1 and B2:
+//   before B1: the wave owning block k's columns applies block k-1's update
+//              to those two tile pairs in temporaries (the same MFMAs in the
+//              same order as its full update below, so the values are
+//              identical) and publishes them;
+//   B1 .. B2:  the deciders decide block k while the field waves apply block
+//              k-1's update to all their pairs;
+//   after B2:  block k's coefficient operands are built from its flip words.
+// So each block costs about the larger of its decisions and its update
+// stream, not their sum.  Same results as k_dense_mfma, bit for bit.
+template <typename FT, int NS, int RULE>
+__global__ __launch_bounds__(64 * MW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_dense_mfma_split(SharedArgs a, const char* __restrict__ Jpk) {
+  typedef MfmaKind<FT> K;
+  typedef typename K::acc_t acc_t;
+  typedef typename K::JT JT;
+  typedef typename ColThresh<FT>::type TT;
+  constexpr int SW = 4;               // field (streaming) waves; waves SW.. decide
+  constexpr int Npad = 64 * NS;
+  constexpr int NP = 2 * NS;          // tile pairs (32 columns each)
+  constexpr int TP = NP / SW;         // tile pairs per field wave
+  constexpr int BPW = NS / SW;        // blocks per field wave (2 pairs each)
+  constexpr int KQ = 64 / K::KH;      // K steps per block
+  constexpr int KP = KQ * K::PARTS;   // B operands per tile pair per block
+  constexpr int EL = K::KH / 2;       // K elements per lane
+  constexpr int NI = KP * TP;         // B operands per field wave per block
+  constexpr int GS = 8;               // operands per load group
+  constexpr int NG = NI / GS;
+  constexpr int TL = 16 / (MW - SW);  // tile loads per deciding wave
+  constexpr int TILE_BYTES = 64 * 64 * (int)sizeof(JT);
+  static_assert(NS % SW == 0 && NI % GS == 0 && NG >= 2 && KP <= GS, "shape");
+  if (a.sel && *a.sel != a.sel_id) return;  // another kernel runs this segment
+
+  extern __shared__ __align__(16) char smem[];
+  char* sD = smem;                                             // diagonal tile
+  FT* sF = reinterpret_cast<FT*>(smem + TILE_BYTES);           // [MG][64]
+  uint64_t* sFlip = reinterpret_cast<uint64_t*>(sF + MG * 64);  // [MG]
+  uint64_t* sSign = sFlip + MG;                                // [MG] old signs
+  uint64_t* sDown = sSign + MG;                                // [MG][NS] spin words
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool fw = w < SW;  // field wave
+  const int b = blockIdx.x;
+  const int n = a.n;
+  const int rbase = b * MG;
+  const JT* J = reinterpret_cast<const JT*>(a.J);
+
+  for (int p = w; p < MG * NS; p += MW) {
+    const int g = p / NS, kb = p - g * NS;
+    const int col = 64 * kb + lane;
+    bool dn = false;
+    if (rbase + g < a.R && col < n) dn = a.spins[(int64_t)(rbase + g) * n + col] < 0;
+    const uint64_t m = ballot64(dn);
+    if (lane == 0) sDown[p] = m;
+  }
+  if (a.T > 0 && !fw) {
+    typename TileVec<JT>::type tile[TL];
+    load_tile<JT, NS, MW - SW>(tile, reinterpret_cast<const char*>(J), 0, w - SW, lane);
+    store_tile<JT, MW - SW>(tile, sD, w - SW, lane);
+  }
+  __syncthreads();
+
+  const int64_t total = (int64_t)a.T * NS;  // blocks of the run
+  uint64_t nflip = 0, nrow = 0;             // workgroup-uniform counters (wave 0)
+  if (fw) {
+    // ================= field waves: tile pairs [w TP, (w+1) TP)
+    const int fbytes_all = a.R * Npad * (int)sizeof(FT);
+    int foff[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int g = 4 * ((lane >> 4) & 1) + j;
+      foff[j] = (rbase + g < a.R)
+                    ? ((rbase + g) * Npad + 32 * w * TP + 16 * (lane >> 5) + (lane & 15)) *
+                          (int)sizeof(FT)
+                    : fbytes_all;
+    }
+    acc_t acc[TP];
+    {
+      const __amdgpu_buffer_rsrc_t frs = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<void*>(a.F_in), (short)0, fbytes_all, 0x00020000);
+#pragma unroll
+      for (int p = 0; p < TP; ++p) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[p][j] = __builtin_bit_cast(
+              FT, __builtin_amdgcn_raw_buffer_load_b32(frs, foff[j], 128 * p, 0));
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    u32x4 buf[2][GS];
+    u32x4 A[KQ];  // coefficient operands of the block whose update is pending
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<char*>(Jpk), (short)0, (int)((size_t)NS * KP * NP * 1024), 0x00020000);
+    const int voff = lane * 16 + w * TP * 1024;
+    // operand (block kb, K step / part kp, local pair p) of this wave
+    auto opnd = [&](int kb, int kp, int p) {
+      const int off =
+          __builtin_amdgcn_readfirstlane(kb * KP * NP * 1024) + (kp * NP + p) * 1024;
+      return __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, voff, off, 0));
+    };
+    auto issue = [&](int kb, int grp, u32x4 (&bb)[GS]) {
+#pragma unroll
+      for (int e = 0; e < GS; ++e) {
+        const int q = grp * GS + e;
+        bb[e] = opnd(kb, q / TP, q - (q / TP) * TP);
+      }
+    };
+    // the pending update (block kb, operands A) on every pair of this wave,
+    // with group 0 already in flight in buf[0]
+    auto stream = [&](int kb) {
+#pragma unroll
+      for (int grp = 0; grp < NG; ++grp) {
+        if (grp + 1 < NG) {
+          issue(kb, grp + 1, buf[(grp + 1) & 1]);
+          __builtin_amdgcn_sched_barrier(0);
+          wait_vm<GS>();
+        } else {
+          wait_vm<0>();
+        }
+#pragma unroll
+        for (int e = 0; e < GS; ++e) {
+          const int q = grp * GS + e;
+          const int kp = q / TP, p = q - kp * TP;
+          acc[p] = mma<FT>(A[kp / K::PARTS], buf[grp & 1][e], acc[p]);
+        }
+      }
+    };
+    int pend = -1;  // block whose update is pending
+#pragma unroll 1
+    for (int64_t gb = 0; gb < total; ++gb) {
+      const int k = (int)(gb % NS);
+      const int ow = k / BPW, lb = k - ow * BPW;
+      // ---- 1: the owner applies the pending update to block k's pairs
+      // (operands in buf, loaded in the previous iteration) and publishes them
+      if (w == ow) {
+        acc_t tmp[2];
+#pragma unroll
+        for (int c = 0; c < BPW; ++c) {
+          if (c != lb) continue;
+          tmp[0] = acc[2 * c];
+          tmp[1] = acc[2 * c + 1];
+        }
+        if (pend >= 0) {
+          wait_vm<0>();
+#pragma unroll
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int kp = 0; kp < KP; ++kp)
+              tmp[h] = mma<FT>(A[kp / K::PARTS], buf[h][kp], tmp[h]);
+          issue(pend, 0, buf[0]);  // group 0 of the pending update stream
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const FT x = tmp[h][j];
+            sF[(4 * ((lane >> 4) & 1) + j) * 64 + 16 * (2 * h + (lane >> 5)) + (lane & 15)] = x;
+          }
+      }
+      bar();  // B1
+      // ---- 3: the pending update on all pairs while block k is decided
+      if (pend >= 0) stream(pend);
+      // operands of the next iteration: block k's update becomes pending
+      const int nk = (k + 1 < NS) ? k + 1 : 0;
+      if (gb + 1 < total && w == nk / BPW) {
+        const int nlb = nk - (nk / BPW) * BPW;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int kp = 0; kp < KP; ++kp) buf[h][kp] = opnd(k, kp, 2 * nlb + h);
+      } else {
+        issue(k, 0, buf[0]);
+      }
+      bar();  // B2
+      // ---- 5: block k's coefficient operands
+      {
+        const int g = lane & 7;
+        const uint64_t flm = sFlip[g], sgm = sSign[g];
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) A[kq] = coef_operand<EL>(flm, sgm, lane, kq);
+      }
+      pend = k;
+      if (w == 0) {
+#pragma unroll
+        for (int g = 0; g < MG; ++g)
+          nflip += (uint64_t)__builtin_popcountll(uniform64(sFlip[g]));
+        const int left = n - 64 * k;  // rows of the block inside the problem
+        nrow += (uint64_t)(left <= 0 ? 0 : (left < 64 ? left : 64));
+      }
+    }
+    if (pend >= 0) stream(pend);  // the last block's update
+    wait_vm<0>();
+    if (a.F_out) {
+      const __amdgpu_buffer_rsrc_t frs = __builtin_amdgcn_make_buffer_rsrc(
+          a.F_out, (short)0, fbytes_all, 0x00020000);
+#pragma unroll
+      for (int p = 0; p < TP; ++p) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          // element copied to a scalar before the bit cast (see k_dense_mfma)
+          const FT x = acc[p][j];
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), frs, foff[j],
+                                                128 * p, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  } else {
+    // ================= deciding waves: replicas g0 = w - SW and g1 = g0 + 4
+    const int dwv = w - SW;
+    const int g0 = dwv, g1 = dwv + 4;
+    const bool act0 = rbase + g0 < a.R, act1 = rbase + g1 < a.R;
+    const uint32_t rep0 = (uint32_t)(a.rep_offset + (act0 ? rbase + g0 : 0));
+    const uint32_t rep1 = (uint32_t)(a.rep_offset + (act1 ? rbase + g1 : 0));
+    const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
+    const int nblocks = gridDim.x;
+    uint32_t done = 0;
+    uint32_t wd0[4] = {0, 0, 0, 0}, wd1[4] = {0, 0, 0, 0};
+    typename TileVec<JT>::type tile[TL];
+#pragma unroll 1
+    for (int64_t gb = 0; gb < total; ++gb) {
+      const int t = (int)(gb / NS), k = (int)(gb - (int64_t)t * NS);
+      const int jj = k & 3;
+      if (jj == 0) {
+        const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
+        const uint32_t c0 = (uint32_t)(16 * (k + (lane & 3)) + (lane >> 2));
+        philox4x32_10(c0, tg, rep0, 0u, slo, shi, wd0[0], wd0[1], wd0[2], wd0[3]);
+        quad_transpose(wd0[0], wd0[1], wd0[2], wd0[3], lane);
+        philox4x32_10(c0, tg, rep1, 0u, slo, shi, wd1[0], wd1[1], wd1[2], wd1[3]);
+        quad_transpose(wd1[0], wd1[1], wd1[2], wd1[3], lane);
+      }
+      bar();  // B1
+      // ---- 3: decide block k for replicas g0 and g1
+      TT nb;
+      if constexpr (sizeof(TT) == 8) nb = a.p0_d[t];
+      else nb = a.p0_f[t];
+      uint64_t flips0 = 0, flips1 = 0;
+      const uint64_t sb0 = uniform64(sDown[g0 * NS + k]);
+      const uint64_t sb1 = uniform64(sDown[g1 * NS + k]);
+      if (act0 || act1) {
+        const JT* D = reinterpret_cast<const JT*>(sD);
+        FieldVec<FT, 64> Dc;  // Dc[i] = D[i][lane]
+#pragma unroll
+        for (int q = 0; q < 64; ++q) Dc.set(q, (FT)D[q * 64 + lane]);
+        __builtin_amdgcn_sched_barrier(0);
+        const int i = 64 * k + lane;
+        const TT th0 =
+            (act0 && i < n) ? accept_log<TT>(pick4(wd0, jj), RULE) * nb : neg_inf<TT>();
+        const TT th1 =
+            (act1 && i < n) ? accept_log<TT>(pick4(wd1, jj), RULE) * nb : neg_inf<TT>();
+        uint64_t live0 = act0 ? ballot64(i < n) : 0ull;
+        uint64_t live1 = act1 ? ballot64(i < n) : 0ull;
+        // within the block every live lane keeps its sign: dE = s2 * F
+        const FT s20 = ((sb0 >> lane) & 1ull) ? (FT)2 : (FT)(-2);
+        const FT s21 = ((sb1 >> lane) & 1ull) ? (FT)2 : (FT)(-2);
+        FT fk0 = sF[g0 * 64 + lane];
+        FT fk1 = sF[g1 * 64 + lane];
+        while (true) {
+          const uint64_t cand0 = ballot64((TT)(s20 * fk0) < th0) & live0;
+          const uint64_t cand1 = ballot64((TT)(s21 * fk1) < th1) & live1;
+          if ((cand0 | cand1) == 0ull) break;
+          if (cand0) {
+            const uint64_t low = cand0 & (~cand0 + 1ull);
+            live0 &= ~(cand0 ^ (cand0 - 1ull));
+            flips0 |= low;
+            const FT c = coef2<FT>((sb0 & low) != 0ull);
+            fk0 = fma_ft<FT>(c, Dc.dyn(__builtin_ctzll(cand0)), fk0);
+          }
+          if (cand1) {
+            const uint64_t low = cand1 & (~cand1 + 1ull);
+            live1 &= ~(cand1 ^ (cand1 - 1ull));
+            flips1 |= low;
+            const FT c = coef2<FT>((sb1 & low) != 0ull);
+            fk1 = fma_ft<FT>(c, Dc.dyn(__builtin_ctzll(cand1)), fk1);
+          }
+        }
+      }
+      if (lane == 0) {
+        sFlip[g0] = flips0;
+        sSign[g0] = sb0;
+        sDown[g0 * NS + k] = sb0 ^ flips0;
+        sFlip[g1] = flips1;
+        sSign[g1] = sb1;
+        sDown[g1 * NS + k] = sb1 ^ flips1;
+      }
+      // soft XCD pacing (performance only, bounded), by the first deciding
+      // wave once its decisions are made (the field waves stream meanwhile)
+      if (dwv == 0 && a.xcd_window > 0 && done > (uint32_t)a.xcd_window) {
+        const uint32_t need = done - (uint32_t)a.xcd_window;
+        for (int spin = 0; spin < a.xcd_spin; ++spin) {
+          if (peer_min(a.progress, b, nblocks, lane) >= need) break;
+          __builtin_amdgcn_s_sleep(4);
+        }
+      }
+      // the next block's diagonal tile
+      const int kn = (k + 1 < NS) ? k + 1 : 0;
+      load_tile<JT, NS, MW - SW>(tile, reinterpret_cast<const char*>(J), kn, dwv, lane);
+      bar();  // B2
+      wait_vm<0>();
+      store_tile<JT, MW - SW>(tile, sD, dwv, lane);
+      if (dwv == 0) {
+        ++done;
+        if (a.xcd_window > 0 && lane == 0)
+          __hip_atomic_store(a.progress + b, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+  }
+  __syncthreads();
+
+  if (a.stats && tid == 0) {
+    atomicAdd(a.stats, (unsigned long long)nflip);
+    atomicAdd(a.stats + 1, (unsigned long long)nrow);
+  }
+  for (int p = w; p < MG * NS; p += MW) {
+    const int g = p / NS, kb = p - g * NS;
+    const int col = 64 * kb + lane;
+    if (rbase + g < a.R && col < n)
+      a.spins[(int64_t)(rbase + g) * n + col] = ((sDown[p] >> lane) & 1ull) ? (int8_t)-1 : (int8_t)1;
+  }
+}
+
 __global__ void k_sweep_select(const unsigned long long* __restrict__ stats, int32_t* sel,
                                uint64_t attempts, int permille, int init) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -467,9 +793,9 @@ hipError_t launch_pack_mfma(int ftype, const void* Jhi, const void* Jlo, int NS,
 
 namespace {
 template <typename FT, int NS, int RULE>
-hipError_t launch_mfma_t(const SharedArgs& a, const void* Jpk, hipStream_t st) {
+hipError_t launch_mfma_t(const SharedArgs& a, const void* Jpk, int form, hipStream_t st) {
   const size_t lds = mfma_sweep_lds_bytes(NS, 4, (int)sizeof(typename MfmaKind<FT>::JT));
-  auto kern = k_dense_mfma<FT, NS, RULE>;
+  auto kern = form ? k_dense_mfma_split<FT, NS, RULE> : k_dense_mfma<FT, NS, RULE>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -483,29 +809,29 @@ hipError_t launch_mfma_t(const SharedArgs& a, const void* Jpk, hipStream_t st) {
 }
 
 template <typename FT, int RULE>
-hipError_t launch_mfma_r(const SharedArgs& a, const void* Jpk, int NS, hipStream_t st) {
+hipError_t launch_mfma_r(const SharedArgs& a, const void* Jpk, int NS, int form, hipStream_t st) {
   switch (NS) {
-    case 16: return launch_mfma_t<FT, 16, RULE>(a, Jpk, st);
-    case 32: return launch_mfma_t<FT, 32, RULE>(a, Jpk, st);
-    case 64: return launch_mfma_t<FT, 64, RULE>(a, Jpk, st);
+    case 16: return launch_mfma_t<FT, 16, RULE>(a, Jpk, form, st);
+    case 32: return launch_mfma_t<FT, 32, RULE>(a, Jpk, form, st);
+    case 64: return launch_mfma_t<FT, 64, RULE>(a, Jpk, form, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <typename FT>
-hipError_t launch_mfma_f(const SharedArgs& a, const void* Jpk, int NS, hipStream_t st) {
-  if (a.rule == RULE_METROPOLIS) return launch_mfma_r<FT, RULE_METROPOLIS>(a, Jpk, NS, st);
-  if (a.rule == RULE_HEAT_BATH) return launch_mfma_r<FT, RULE_HEAT_BATH>(a, Jpk, NS, st);
+hipError_t launch_mfma_f(const SharedArgs& a, const void* Jpk, int NS, int form, hipStream_t st) {
+  if (a.rule == RULE_METROPOLIS) return launch_mfma_r<FT, RULE_METROPOLIS>(a, Jpk, NS, form, st);
+  if (a.rule == RULE_HEAT_BATH) return launch_mfma_r<FT, RULE_HEAT_BATH>(a, Jpk, NS, form, st);
   return hipErrorInvalidValue;
 }
 }  // namespace
 
-hipError_t launch_dense_mfma(int ftype, const SharedArgs& a, const void* Jpk, int NS,
+hipError_t launch_dense_mfma(int ftype, const SharedArgs& a, const void* Jpk, int NS, int form,
                              hipStream_t st) {
   if (a.R <= 0 || a.T < 0) return hipSuccess;
   if (!mfma_sweep_supported(NS) || !a.F_in || !Jpk) return hipErrorInvalidValue;
-  if (ftype == FT_F32) return launch_mfma_f<float>(a, Jpk, NS, st);
-  if (ftype == FT_I32) return launch_mfma_f<int32_t>(a, Jpk, NS, st);
+  if (ftype == FT_F32) return launch_mfma_f<float>(a, Jpk, NS, form, st);
+  if (ftype == FT_I32) return launch_mfma_f<int32_t>(a, Jpk, NS, form, st);
   return hipErrorInvalidValue;
 }
 

@@ -92,7 +92,8 @@ struct ojmi_ctx {
   // (2: per 16-sweep segment, by the acceptance rate the GPU measured in the
   // previous one, against the column-owned kernel)
   int mfma_sweep = 0;
-  int mfma_accept = 110;  // per mille: the matrix-core sweep at or above this rate
+  int mfma_accept = 85;  // per mille: the matrix-core sweep at or above this rate
+  int mfma_form = 1;     // 1: four waves update the fields while four decide; 0: all do both
   void* dJpk = nullptr;  // packed MFMA operands of the couplings (k_dense_mfma)
   int32_t* dsel = nullptr;  // segment kernel choice and flip snapshot (mode 2)
   int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
@@ -750,7 +751,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     }
     c->lds_used = (int)mfma_sweep_lds_bytes(c->NS, 4, c->ftype == FT_F32 ? 4 : 1);
     c->ring_rows_used = 0;
-    if (!hybrid_sweep(c)) return launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->st);
+    if (!hybrid_sweep(c)) return launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->mfma_form, c->st);
     // one segment: the kernel the selector chose runs, the other returns at once
     hipError_t e;
     if (t_begin == 0 &&
@@ -764,7 +765,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     if (sa.xcd_window > 0 &&
         (e = hipMemsetAsync(c->dprog, 0, (size_t)nwg * 4, c->st)) != hipSuccess)
       return e;
-    if ((e = launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->st)) != hipSuccess) return e;
+    if ((e = launch_dense_mfma(c->ftype, sa, c->dJpk, c->NS, c->mfma_form, c->st)) != hipSuccess) return e;
     const uint64_t attempts = (uint64_t)c->R * (uint64_t)c->n * (uint64_t)T;
     return launch_sweep_select(c->dstats, c->dsel, attempts, c->mfma_accept, 0, c->st);
   }
@@ -939,11 +940,11 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
     }
   }
   // Large FP32 systems with replicas enough to fill the device: per 16-sweep
-  // segment the matrix-core sweep while at least 11% of the flip attempts are
-  // accepted, the column-owned kernel below (measured on dense SK N = 4096,
-  // 2048 replicas: 1.05 -> 0.66 ms per sweep for the matrix-core sweep from
-  // acceptance 0.91 to 0.005, 2.0 -> 0.45 ms for the column-owned one,
-  // crossing at 0.11)
+  // segment the matrix-core sweep while at least 8.5% of the flip attempts
+  // are accepted, the column-owned kernel below (measured on dense SK
+  // N = 4096, 2048 replicas: 0.79 -> 0.65 ms per sweep for the matrix-core
+  // sweep from acceptance 0.91 to 0.005, 2.0 -> 0.45 ms for the column-owned
+  // one, crossing near 0.085)
   if (c->ftype == FT_F32 && c->M == 1 && c->NS >= 32 && c->R >= 2048 && colown_form(c) &&
       mfma_sweep_shape(c)) {
     if ((rc = dalloc(&c->dsel, 4))) {
@@ -1465,6 +1466,11 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
       if (rc) return rc;
     }
     c->mfma_sweep = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "mfma_form") {
+    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "mfma_form must be 0 or 1");
+    c->mfma_form = (int)value;
     return OJMI_OK;
   }
   if (k == "mfma_accept") {

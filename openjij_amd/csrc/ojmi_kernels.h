@@ -118,7 +118,9 @@ size_t mfma_packed_bytes(int ftype, int NS);
 // FP32: Jhi, Jlo natural bf16 couplings; INT32: Jhi natural int8, Jlo unused.
 hipError_t launch_pack_mfma(int ftype, const void* Jhi, const void* Jlo, int NS, void* out,
                             hipStream_t st);
-hipError_t launch_dense_mfma(int ftype, const SharedArgs& a, const void* Jpk, int NS,
+// form 0: every wave decides and updates; form 1: four waves update the
+// fields while four decide (the same results)
+hipError_t launch_dense_mfma(int ftype, const SharedArgs& a, const void* Jpk, int NS, int form,
                              hipStream_t st);
 // Kernel choice of the next sweep segment from the accepted flips of the last
 // one: sel = 1 (matrix-core sweep) when flips * 1000 >= permille * attempts,

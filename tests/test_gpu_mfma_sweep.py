@@ -37,9 +37,12 @@ def exact_fields(E, spins):
     return spins.astype(np.float64) @ S[:n, :n] + S[:n, n]
 
 
-def mfma_ctx(E, R, prec):
+def mfma_ctx(E, R, prec, form=0):
+    """form 0: every wave decides and updates; 1: four waves hold the fields
+    and stream the updates while four decide."""
     ctx = capi.Context.dense(E, R, 1, 0.0, PREC[prec])
     ctx.set_option("mfma_sweep", 1)
+    ctx.set_option("mfma_form", form)
     info = ctx.info()
     assert info["kernel"] == capi.KERNEL_DENSE_MFMA and info["group"] == 8
     return ctx
@@ -48,12 +51,13 @@ def mfma_ctx(E, R, prec):
 @pytest.mark.parametrize("n,R,L", [(1000, 8, 10), (1025, 10, 8), (2048, 17, 6), (4096, 16, 4),
                                    (3000, 3, 5), (961, 9, 12)])
 @pytest.mark.parametrize("rule", [0, 1])
-def test_int32_bitexact(n, R, L, rule):
-    print(f"mfma sweep int32: n={n} R={R} sweeps={L} rule={rule}", flush=True)
+@pytest.mark.parametrize("form", [0, 1])
+def test_int32_bitexact(n, R, L, rule, form):
+    print(f"mfma sweep int32: n={n} R={R} sweeps={L} rule={rule} form={form}", flush=True)
     E = H.int_dense(n, 0x3A00 + n)
     beta, mcs = H.geometric_betas(0.05, 3.0, L), np.ones(L, dtype=np.int64)
     init = H.random_pm1(R, n, n + 3)
-    with mfma_ctx(E, R, "int32") as ctx:
+    with mfma_ctx(E, R, "int32", form) as ctx:
         ctx.set_option("update_method", rule)
         ctx.set_spins(init)
         ctx.run(beta, mcs, 0x5EED, 7)
@@ -69,14 +73,15 @@ def test_int32_bitexact(n, R, L, rule):
     assert info["last_rows"] == ((R + 7) // 8) * L * n
 
 
-def test_int32_split_run_and_pacing():
+@pytest.mark.parametrize("form", [0, 1])
+def test_int32_split_run_and_pacing(form):
     """Two runs with consecutive sweep offsets equal one run; 32 workgroups,
     so XCD pacing is active (performance only)."""
     n, R = 1024, 256
     E = H.int_dense(n, 0x3B00)
     beta, mcs = H.geometric_betas(0.1, 4.0, 12), np.ones(12, dtype=np.int64)
     init = H.random_pm1(R, n, 11)
-    with mfma_ctx(E, R, "int32") as ctx:
+    with mfma_ctx(E, R, "int32", form) as ctx:
         ctx.set_spins(init)
         ctx.run(beta[:5], mcs[:5], 99, 0)
         ctx.run(beta[5:], mcs[5:], 99, 5)
@@ -87,7 +92,8 @@ def test_int32_split_run_and_pacing():
 
 
 @pytest.mark.parametrize("n,R", [(4096, 16), (1024, 24), (2000, 8)])
-def test_fp32_fields_and_trajectories(n, R):
+@pytest.mark.parametrize("form", [0, 1])
+def test_fp32_fields_and_trajectories(n, R, form):
     """FP32: fields after a run against the exact fields of the final spins,
     and trajectories against the FP32 replay from the same initial fields."""
     print(f"mfma sweep fp32: n={n} R={R}", flush=True)
@@ -99,7 +105,7 @@ def test_fp32_fields_and_trajectories(n, R):
         ref.set_spins(init)
         ref.refresh_fields()
         F0 = ref.get_fields()
-    with mfma_ctx(E, R, "fp32") as ctx:
+    with mfma_ctx(E, R, "fp32", form) as ctx:
         ctx.set_spins(init)
         ctx.run(beta, mcs, 0xF00D, 0)
         got = ctx.get_spins()
