@@ -47,6 +47,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK = 8.0e12       # B/s, MI355X HBM3E
 VALU_PEAK = 78.6e12     # FP32 / INT32 / FP64 vector ops/s without packed FP32 (disabled here)
 MFMA_BF16_PEAK = 2.5e15  # dense bf16 MFMA flop/s
+# L2 -> CU read bandwidth: 16 channels x 128 B/clk per XCD, 8 XCDs, 2.4 GHz
+L2_PEAK = 16 * 128 * 8 * 2.4e9
 MFMA_F64_PEAK = 78.6e12  # FP64 MFMA flop/s
 CPU_THREADS = 16        # host-core share of one GPU on the measurement machine
 XCDS = 8                # L2 domains of the MI355X
@@ -350,7 +352,9 @@ def roofline(W, info, sweep_ms, launches, default_run):
       (N fused multiply-adds) per accepted flip, counted on the GPU; CSR
       recomputes each field from its deg neighbours at every attempt.
     l2: bytes of coupling rows the workgroups stream (each row once per
-      workgroup per 64-spin block it was flipped in, counted on the GPU).
+      workgroup per 64-spin block it was flipped in, counted on the GPU; the
+      matrix-core sweep streams all 64 rows of every block), against the L2
+      read bandwidth: what binds the matrix-core sweep.
     """
     bJ = info["coupling_bytes"]
     T = int(W["mcs"].sum())
@@ -384,12 +388,13 @@ def roofline(W, info, sweep_ms, launches, default_run):
         "traffic": None if traffic is None else traffic / launches,
         "alg_bytes_per_launch": alg / launches,
         "launch_ms": t * 1e3, "launches": launches,
-        "binding": "valu",
+        "binding": "l2" if info["kernel"] == 10 else "valu",
         "valu": {"achieved": a_valu / 1e12, "peak": VALU_PEAK / 1e12, "unit": "TFLOP/s",
                  "frac": a_valu / VALU_PEAK, "flops_per_launch": flop / launches,
                  "accepted_flips": flips},
         "l2_rows": {"rows_streamed": rows, "bytes_per_launch": row_bytes / launches,
-                    "GB_per_s": row_bytes / launches / t / 1e9},
+                    "GB_per_s": row_bytes / launches / t / 1e9, "peak_GB_per_s": L2_PEAK / 1e9,
+                    "frac": row_bytes / launches / t / L2_PEAK},
     }
 
 
