@@ -601,12 +601,15 @@ bool mfma_sweep_shape(const ojmi_ctx* c) {
          ((c->ftype == FT_F32 && c->dJhi) || (c->ftype == FT_I32 && c->jtype == JT_I8 && c->dJnat)) &&
          c->R * c->Npad * 4 < ((int64_t)1 << 31);
 }
-bool uses_mfma_sweep(const ojmi_ctx* c) { return c->mfma_sweep && mfma_sweep_shape(c); }
 // Mode 2 alternates with the column-owned kernel, which needs the
-// workgroup-shared family in its column-owned form.
+// workgroup-shared family in its column-owned form; with another launch
+// shape chosen (group, colown) that shape runs instead.
 bool colown_form(const ojmi_ctx* c) {
   return c->family == OJMI_KERNEL_DENSE_SHARED && c->colown &&
          colown_supported((int)c->NS, c->group) && c->rule == RULE_METROPOLIS;
+}
+bool uses_mfma_sweep(const ojmi_ctx* c) {
+  return mfma_sweep_shape(c) && (c->mfma_sweep == 1 || (c->mfma_sweep == 2 && colown_form(c)));
 }
 bool hybrid_sweep(const ojmi_ctx* c) { return c->mfma_sweep == 2 && uses_mfma_sweep(c); }
 
@@ -939,19 +942,22 @@ int ojmi_create_dense(int64_t n, const double* J_ext, int64_t num_replicas, int6
       break;
     }
   }
-  // Large FP32 systems with replicas enough to fill the device: per 16-sweep
-  // segment the matrix-core sweep while at least 8.5% of the flip attempts
-  // are accepted, the column-owned kernel below (measured on dense SK
-  // N = 4096, 2048 replicas: 0.79 -> 0.65 ms per sweep for the matrix-core
-  // sweep from acceptance 0.91 to 0.005, 2.0 -> 0.45 ms for the column-owned
-  // one, crossing near 0.085)
-  if (c->ftype == FT_F32 && c->M == 1 && c->NS >= 32 && c->R >= 2048 && colown_form(c) &&
+  // Large FP32 systems with replicas enough to fill the device (8 per
+  // workgroup): per 16-sweep segment the matrix-core sweep while the
+  // acceptance rate is at least the autotuner's crossover for this size, the
+  // column-owned kernel below (ojmi_tuned.h)
+  if (c->ftype == FT_F32 && c->M == 1 && c->R >= 8 * (int64_t)c->num_cus && colown_form(c) &&
       mfma_sweep_shape(c)) {
-    if ((rc = dalloc(&c->dsel, 4))) {
-      delete c;
-      return rc;
+    for (const tuned::MfmaChoice& e : tuned::kMfma) {
+      if (e.slots != c->NS || e.accept_permille <= 0) continue;
+      if ((rc = dalloc(&c->dsel, 4))) {
+        delete c;
+        return rc;
+      }
+      c->mfma_sweep = 2;
+      c->mfma_accept = e.accept_permille;
+      break;
     }
-    c->mfma_sweep = 2;
   }
   *out = c;
   return OJMI_OK;

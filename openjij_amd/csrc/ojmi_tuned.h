@@ -1,5 +1,5 @@
 // ojmi_tuned.h -- launch shapes chosen by openjij_amd/tune.py (generated;
-// measured on an AMD Instinct MI355X, gfx950).  Re-run the tuner on another GPU and rebuild.
+// measured on AMD Instinct MI355X).  Re-run the tuner on another GPU and rebuild.
 #pragma once
 #include <stdint.h>
 
@@ -17,7 +17,7 @@ constexpr CsrShape kCsr[] = {
     {0, 362, 4},
     {0, 724, 4},
     {0, 1448, 2},
-    {0, 2896, 2},
+    {0, 2896, 1},
     {0, 1099511627776, 1},
     {1, 362, 4},
     {1, 724, 4},
@@ -73,6 +73,19 @@ constexpr DenseShape kDense[] = {
 // Batches of problems: default launch (0 fused largest first, 1 per slot
 // class, 2 persistent per-XCD queues).
 constexpr int kBatchMode = 0;
+
+// Dense FP32 systems with replicas to fill the device: per 16-sweep
+// segment the matrix-core sweep while the acceptance rate is at least
+// accept_permille per mille, the column-owned kernel below (0: never the
+// matrix-core sweep), by 64-spin slot count.
+struct MfmaChoice {
+  int slots;
+  int accept_permille;
+};
+constexpr MfmaChoice kMfma[] = {
+    {32, 66},
+    {64, 87},
+};
 
 }  // namespace tuned
 }  // namespace ojmi

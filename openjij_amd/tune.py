@@ -16,7 +16,13 @@ Buckets and candidates:
     and without XCD pacing; rows staged in LDS, G = 8 / 16), per field type,
     slot count (64-spin slots) and replica count;
   * batches of uneven problems: one launch largest first, one launch per slot
-    class, or one persistent launch over per-XCD queues.
+    class, or one persistent launch over per-XCD queues;
+  * large dense FP32 systems (2048 replicas; N = 2048, 4096): a geometric
+    anneal run 16 sweeps at a time on the matrix-core sweep and on the
+    column-owned kernel, each segment's acceptance rate counted on the GPU;
+    the table keeps the acceptance rate at which the two cross (the
+    per-segment kernel choice of ``mfma_sweep`` 2), or 0 when the matrix-core
+    sweep never wins.
 
   python -m openjij_amd.tune [--quick]
 """
@@ -152,6 +158,7 @@ def tune_dense(capi, quick):
                 spins = _pm1((R, n), n + R)
                 times = {}
                 with capi.Context.dense(E, R, 1, 0.0, prec) as ctx:
+                    ctx.set_option("mfma_sweep", 0)  # the per-flip kernels' own shapes
                     for G, col, win in cands:
                         try:
                             ctx.set_option("group", G)
@@ -216,7 +223,55 @@ def tune_batch(capi, quick):
     return {"problems": K, "replicas": R, "sweeps": L, "ms": times, "best": best}, best
 
 
-def write_header(path, device, csr, dense, batch_mode):
+def tune_mfma(capi, quick):
+    K, R = 16, 2048
+    L = 160 if quick else 500
+    beta = _geometric(0.1, 10.0, L)
+    rows, table = [], []
+    for n in ((4096,) if quick else (2048, 4096)):
+        ns = 1 << max(0, (n + 63) // 64 - 1).bit_length()
+        E = _sk(n, 0xAC + n, False)
+        spins = _pm1((R, n), n + 7)
+        seg = {}
+        for mode in (0, 1):
+            with capi.Context.dense(E, R, 1, 0.0, capi.PREC_FP32) as ctx:
+                ctx.set_option("group", 4)
+                ctx.set_option("colown", 1)
+                ctx.set_option("refresh_every", K)
+                ctx.set_option("mfma_sweep", mode)
+                ctx.set_spins(spins)
+                out = []
+                for c in range(0, L, K):
+                    b = beta[c:c + K]
+                    ctx.run(b, np.ones(len(b), np.int64), 3, c)
+                    i = ctx.info()
+                    out.append((i["last_flips"] / (R * n * len(b)),
+                                (i["last_kernel_ms"] - i["last_refresh_ms"]) / len(b)))
+                seg[mode] = out
+        # segments of equal acceptance (the same spins, seed and schedule up to
+        # the kernels' rounding): the crossover is where the matrix-core sweep
+        # stops winning as the acceptance falls
+        pairs = [(a, t0, t1) for (a, t0), (_, t1) in zip(seg[0], seg[1])]
+        wins = [a for a, t0, t1 in pairs if t1 < t0]
+        if not wins:
+            permille = 0
+        else:
+            lo = min(wins)
+            below = [a for a, t0, t1 in pairs if a < lo]
+            hi = max(below) if below else 0.0
+            permille = max(1, int(round(500.0 * (lo + hi))))
+        rows.append({"n": n, "slots": ns, "R": R,
+                     "segments": [{"accept": a, "colown_ms_per_sweep": t0,
+                                   "mfma_ms_per_sweep": t1} for a, t0, t1 in pairs],
+                     "accept_permille": permille})
+        print(f"mfma n={n}: crossover at {permille} per mille "
+              f"(colown {sum(p[1] for p in pairs) * K:.1f} ms, "
+              f"matrix cores {sum(p[2] for p in pairs) * K:.1f} ms per anneal)", flush=True)
+        table.append((ns, permille))
+    return rows, table
+
+
+def write_header(path, device, csr, dense, batch_mode, mfma):
     lines = [
         "// ojmi_tuned.h -- launch shapes chosen by openjij_amd/tune.py (generated;",
         f"// measured on {device}).  Re-run the tuner on another GPU and rebuild.",
@@ -276,6 +331,21 @@ def write_header(path, device, csr, dense, batch_mode):
         "// class, 2 persistent per-XCD queues).",
         f"constexpr int kBatchMode = {batch_mode};",
         "",
+        "// Dense FP32 systems with replicas to fill the device: per 16-sweep",
+        "// segment the matrix-core sweep while the acceptance rate is at least",
+        "// accept_permille per mille, the column-owned kernel below (0: never the",
+        "// matrix-core sweep), by 64-spin slot count.",
+        "struct MfmaChoice {",
+        "  int slots;",
+        "  int accept_permille;",
+        "};",
+        "constexpr MfmaChoice kMfma[] = {",
+    ]
+    for ns, pm in sorted(mfma):
+        lines.append(f"    {{{ns}, {pm}}},")
+    lines += [
+        "};",
+        "",
         "}  // namespace tuned",
         "}  // namespace ojmi",
         "",
@@ -298,7 +368,10 @@ def main():
     try:  # the device name, from the HIP runtime the engine links
         import ctypes
 
-        hip = ctypes.CDLL("libamdhip64.so")
+        try:
+            hip = ctypes.CDLL("libamdhip64.so")
+        except OSError:
+            hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
         buf = ctypes.create_string_buffer(256)
         if hip.hipInit(0) == 0 and hip.hipDeviceGetName(buf, 256, 0) == 0:
             device = buf.value.decode() or device
@@ -308,10 +381,11 @@ def main():
     csr_rows, csr = tune_csr(capi, args.quick)
     dense_rows, dense = tune_dense(capi, args.quick)
     batch_row, batch_mode = tune_batch(capi, args.quick)
+    mfma_rows, mfma = tune_mfma(capi, args.quick)
     with open(args.json, "w") as f:
         json.dump({"device": device, "seconds": time.time() - t0, "csr": csr_rows,
-                   "dense": dense_rows, "batch": batch_row}, f, indent=1)
-    write_header(args.out, device, csr, dense, batch_mode)
+                   "dense": dense_rows, "batch": batch_row, "mfma": mfma_rows}, f, indent=1)
+    write_header(args.out, device, csr, dense, batch_mode, mfma)
     print(f"wrote {args.out} and {args.json} ({time.time() - t0:.0f} s)", flush=True)
 
 
