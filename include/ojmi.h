@@ -60,6 +60,7 @@ extern "C" {
 #define OJMI_KERNEL_BATCH_PERSIST 8 /* batch of problems, one persistent launch, LPT queues */
 #define OJMI_KERNEL_BATCH_FUSED 9   /* batch of problems, one launch, largest problem first */
 #define OJMI_KERNEL_DENSE_MFMA 10   /* 8 replicas per workgroup, row updates on the matrix cores */
+#define OJMI_KERNEL_SQA_CSR 11      /* Trotter CSR: one workgroup per replica, a wave per slice */
 
 typedef struct ojmi_ctx ojmi_ctx;
 

@@ -19,7 +19,7 @@ OK, EINVAL, ERANGE, EDEVICE, ENOMEM, ENOTSUP, ERUNTIME = range(7)
 PREC_AUTO, PREC_FP64, PREC_FP32, PREC_INT32 = range(4)
 KERNEL_DENSE_WAVE, KERNEL_SQA_WAVE, KERNEL_CSR_LEVEL, KERNEL_DENSE_SHARED = 1, 2, 3, 4
 KERNEL_DENSE_COLOWN, KERNEL_BATCH_WAVE, KERNEL_DENSE_LDS, KERNEL_BATCH_PERSIST = 5, 6, 7, 8
-KERNEL_BATCH_FUSED, KERNEL_DENSE_MFMA = 9, 10
+KERNEL_BATCH_FUSED, KERNEL_DENSE_MFMA, KERNEL_SQA_CSR = 9, 10, 11
 
 
 class Info(C.Structure):

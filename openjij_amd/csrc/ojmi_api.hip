@@ -702,6 +702,33 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.stats = c->dstats;
     return launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   }
+  if (c->family == OJMI_KERNEL_SQA_CSR) {
+    CsrArgs a{};
+    a.order = c->d_order;
+    a.lvl_ptr = c->d_lvl;
+    a.nlev = c->nlev;
+    a.deg = c->d_deg;
+    a.ell_col = c->d_ell;
+    a.val = c->d_ellval;
+    a.D = c->D;
+    a.h = c->dh;
+    a.spins = c->dspins;
+    a.n = (int)c->n;
+    a.R = (int)c->R;
+    a.M = (int)c->M;
+    a.T = (int)T;
+    a.seed = seed;
+    a.t0 = sweep_offset + (uint64_t)t_begin;
+    a.rep_offset = 0;
+    a.p0_d = c->dp0_d + t_begin;
+    a.p0_f = c->dp0_f + t_begin;
+    a.p1_d = c->dp1_d + t_begin;
+    a.p1_f = c->dp1_f + t_begin;
+    a.rule = RULE_METROPOLIS;
+    a.stats = c->dstats;
+    c->lds_used = (int)sqa_csr_lds_bytes((int)c->n, (int)c->M);
+    return launch_sqa_csr(c->ftype, a, c->st);
+  }
   void* F_out = carries_fields(c) ? c->dF : nullptr;
   DenseArgs a{};
   a.J = c->dJ;
@@ -1040,9 +1067,12 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   pick_types(c, maxabsJ);
   if (c->jtype == JT_I8) c->jtype = JT_I32;
 
-  // Trotter systems and graphs with a node of degree > 64 (the widest
-  // neighbour list of the CSR kernel) run on the dense kernels.
-  const bool dense_like = (num_trotter > 1) || (D > 64);
+  // Graphs with a node of degree > 64 (the widest neighbour list of the CSR
+  // kernels) run on the dense kernels, and so do Trotter systems whose slices'
+  // spins do not fit the Trotter CSR kernel's LDS.
+  const bool sqa_csr = num_trotter > 1 && D <= 64 &&
+                       sqa_csr_lds_bytes((int)n, (int)num_trotter) <= (size_t)device_lds_budget();
+  const bool dense_like = (num_trotter > 1 && !sqa_csr) || (D > 64);
   if (dense_like) {
     std::vector<double> Jn((size_t)(n * n), 0.0);
     for (int64_t i = 0; i < n; ++i)
@@ -1088,7 +1118,7 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
       ellv[d * n + pos] = adj[i][d].second;
     }
   }
-  c->family = OJMI_KERNEL_CSR_LEVEL;
+  c->family = sqa_csr ? OJMI_KERNEL_SQA_CSR : OJMI_KERNEL_CSR_LEVEL;
   c->nlev = nlev;
   c->D = (int)Dp;
   c->NS = pow2_slots(n);
@@ -1096,7 +1126,7 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   // per wave: Philox words (16 B per 4 spins) + int8 spins; <= 64 KiB per block
   const size_t per_wave = (size_t)((n + 3) / 4) * 20;
   c->csr_wpb = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per_wave));
-  if (per_wave > 65536) {
+  if (per_wave > 65536 && !sqa_csr) {
     delete c;
     return fail(OJMI_ENOTSUP, "CSR GPU path: n too large for LDS-resident spins");
   }
@@ -1631,7 +1661,7 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
   OJMI_HIP(hipMemcpyAsync(c->dp0_f, p0f.data(), T * 4, hipMemcpyHostToDevice, c->st));
   OJMI_HIP(hipMemcpyAsync(c->dp1_f, p1f.data(), T * 4, hipMemcpyHostToDevice, c->st));
   if (c->family != OJMI_KERNEL_CSR_LEVEL && c->family != OJMI_KERNEL_SQA_WAVE &&
-      (rc = ensure_fields(c)))
+      c->family != OJMI_KERNEL_SQA_CSR && (rc = ensure_fields(c)))
     return rc;
   const bool msweep = uses_mfma_sweep(c);
   if ((c->family == OJMI_KERNEL_DENSE_SHARED || msweep) && (rc = ensure_progress(c))) return rc;
@@ -1641,7 +1671,8 @@ int ojmi_run(ojmi_ctx* c, const double* beta, const double* s, const int64_t* mc
 
   OJMI_HIP(hipEventRecord(c->e0, c->st));
   hipError_t e = hipSuccess;
-  if (c->family == OJMI_KERNEL_CSR_LEVEL || c->family == OJMI_KERNEL_SQA_WAVE) {
+  if (c->family == OJMI_KERNEL_CSR_LEVEL || c->family == OJMI_KERNEL_SQA_WAVE ||
+      c->family == OJMI_KERNEL_SQA_CSR) {
     e = launch_sweeps(c, 0, T, seed, sweep_offset, nullptr);
   } else {
     // Fields: from the matrix cores (exact int8 for INT32 couplings; bf16

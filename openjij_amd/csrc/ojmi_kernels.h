@@ -68,6 +68,8 @@ struct CsrArgs {
   const float* p0_f;
   int rule;
   unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
+  const double* p1_d = nullptr;  // Trotter: B per sweep (p0 = s beta / M)
+  const float* p1_f = nullptr;
 };
 
 // Workgroup-shared dense sweep (k_shared.hip): J in natural row layout
@@ -158,6 +160,11 @@ int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int 
 // waves: 0 (choose), 1, 2, 4 or 8 waves per replica.
 hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, int waves,
                             hipStream_t st);
+
+// Trotter sweeps of CSR systems (k_sqa_csr.hip): one workgroup per replica,
+// spins of the M slices in LDS (n * M bytes), a.M >= 2, a.D the ELL width.
+size_t sqa_csr_lds_bytes(int n, int M);
+hipError_t launch_sqa_csr(int ftype, const CsrArgs& a, hipStream_t st);
 
 // Automatic-schedule statistics of every problem of a batch (k_beta_stats.hip):
 // out[k] = {dE_max, dE_min, lin_max, mean_abs, int_min} from the spin pair
