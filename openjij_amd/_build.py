@@ -22,6 +22,7 @@ HIP_SOURCES = [
     "k_sqa_f32_large.hip", "k_sqa_f64_small.hip", "k_sqa_f64_large.hip", "k_shared.hip", "k_colown.hip",
     "k_colown_i8.hip", "k_colown_i32.hip", "k_lds.hip", "k_csr.hip", "k_energy.hip",
     "k_spin_gemm.hip", "k_beta_stats.hip", "k_batch_persist.hip", "k_mfma_sweep.hip", "k_sqa_csr.hip",
+    "k_sqa_wide.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

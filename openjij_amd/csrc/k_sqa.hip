@@ -7,6 +7,7 @@ namespace ojmi {
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W,
                            hipStream_t st) {
   if (a.R <= 0 || a.T < 0) return hipSuccess;
+  if (NS == 64) return launch_sqa_wide(ftype, a, PPW, W, st);
   const bool large = NS >= 16;
   if (ftype == FT_F64)
     return large ? launch_sqa_f64_large(a, NS, PPW, W, st) : launch_sqa_f64_small(a, NS, PPW, W, st);

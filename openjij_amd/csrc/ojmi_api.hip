@@ -374,7 +374,9 @@ int setup_dense_family(ojmi_ctx* c) {
   } else {
     const int64_t P = (c->M + 1) / 2;
     const int words = (c->ftype == FT_F64) ? 2 : 1;
-    if (c->NS > 32) return fail(OJMI_ENOTSUP, "Trotter GPU path supports n <= 2048 spins");
+    // 64 slots (n <= 4096) hold one slice's fields in 64 (FP32) or 128 (FP64)
+    // registers per lane; the register budget below bounds the slices
+    if (c->NS > 64) return fail(OJMI_ENOTSUP, "Trotter GPU path supports n <= 4096 spins");
     auto pick = [&](int wmax, int ppw_max, int budget, int* Wout, int* Pout) -> bool {
       const int W = (int)std::min<int64_t>(P, wmax);
       const int64_t need = (P + W - 1) / W;

@@ -190,5 +190,7 @@ hipError_t launch_sqa_f32_small(const DenseArgs& a, int NS, int PPW, int W, hipS
 hipError_t launch_sqa_f32_large(const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
 hipError_t launch_sqa_f64_small(const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
 hipError_t launch_sqa_f64_large(const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+// NS = 64: FP64 with one slice pair per wave, FP32 with one or two
+hipError_t launch_sqa_wide(int ftype, const DenseArgs& a, int PPW, int W, hipStream_t st);
 
 }  // namespace ojmi

@@ -382,6 +382,42 @@ def test_sqa_dense_bitexact(n, M, prec):
         assert list(sol[r]) == list(got[r, int(np.argmin(en[r]))])
 
 
+@pytest.mark.parametrize("n,M,prec", [(2049, 3, "fp32"), (3000, 8, "fp64"), (4096, 7, "fp64"),
+                                      (4096, 16, "fp32"), (3500, 10, "fp32")])
+def test_sqa_dense_wide_bitexact(n, M, prec):
+    """2049-4096 spins: 64 slots per slice, one slice pair per wave (FP64) or
+    up to two (FP32)."""
+    print(f"dense Trotter wide: n={n} M={M} {prec}", flush=True)
+    E = H.sk_gauss(n, 6000 + n + M, field=True)
+    L = 4
+    s = np.linspace(0.2, 0.8, L)
+    beta = np.full(L, 3.0)
+    mcs = np.ones(L, dtype=np.int64)
+    gamma = 1.0
+    R = 2
+    init = np.stack([H.random_pm1(M, n, 7 + r) for r in range(R)])
+    with capi.Context.dense(E, R, M, gamma, PREC[prec]) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_SQA_WAVE
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 0x3E3E, 2, s=s)
+        got = ctx.get_spins()
+        en = ctx.energies()
+    a, B = orc.expand_transverse(beta, s, mcs, gamma, M)
+    want = orc.replay_sqa_dense(prec, H.symmetric_ext(E), init, 0x3E3E, 2, a, B)
+    np.testing.assert_array_equal(got, want)
+    for r in range(R):
+        ex = orc.energy_exact(E, got[r])
+        assert np.all(np.abs(en[r] - ex) <= 1e-12 * (1 + np.abs(ex)))
+
+
+def test_sqa_dense_wide_limits():
+    """Past the registers of the 64-slot Trotter kernel (FP64 above 8
+    slices, FP32 above 16, any n above 4096) the context is refused."""
+    for n, M, prec in ((3000, 9, "fp64"), (3000, 17, "fp32"), (4097, 2, "fp32")):
+        with pytest.raises(capi.OjmiError):
+            capi.Context.dense(H.sk_gauss(n, 7), 2, M, 1.0, PREC[prec])
+
+
 def test_sqa_config_c_shape_replay():
     """SK N=256 Gaussian, M=32, Gamma=5, s linear to 4.99/5 (a short slice of
     the 1000-sweep schedule), fp32 and fp64."""

@@ -6,7 +6,7 @@ Bit-exact against the oracle's CSR Trotter replay (fields recomputed per
 attempt in stored CSR order, reference include/openjij/system/
 transverse_ising.hpp:426-596 under updater/single_spin_flip.hpp:128-251) for
 integer and Gaussian couplings, FP64 and FP32, odd and even M, the four
-neighbour-list widths, systems beyond the dense Trotter kernel's 2048 spins
+neighbour-list widths, systems beyond the dense Trotter kernel's 4096 spins
 and the largest n*M the LDS holds.  Energies per slice against the exact FP64
 evaluation; the solution is the slice of lowest energy.
 """
@@ -111,7 +111,7 @@ def test_sqa_csr_split_run_equals_one_run():
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_sqa_csr_beyond_dense_limit(prec):
-    """EA3D L = 17 (4913 spins, more than the dense Trotter kernel's 2048),
+    """EA3D L = 17 (4913 spins, more than the dense Trotter kernel's 4096),
     M = 8."""
     n, rp, ci, va = H.ea3d_csr(17, 0x5E17)
     M, R, L, gamma = 8, 2, 3, 1.0
