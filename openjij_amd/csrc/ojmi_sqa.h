@@ -100,23 +100,38 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
               const uint64_t bu = s_down[tu * NS + k], bd = s_down[td * NS + k];
               // s_up + s_dn = 2 - 2(u + d)
               const int nb2 = 2 - 2 * (int)(((bu >> lane) & 1ull) + ((bd >> lane) & 1ull));
+              // A lane leaves `live` when it flips (and so do the lanes below
+              // a flip), so every live lane keeps its own spin, and with it
+              // the Trotter term, for the whole slot: s2 and x = B dEtr are
+              // formed once per slot, dEcl = s2 F per attempt, as before.
+              // (With 8 slice pairs per wave the per-attempt forms keep fewer
+              // values live across the pairs' loops, so they stay there.)
+              const bool dn = (dw >> k) & 1ull;
+              const FT s2 = dn ? (FT)2 : (FT)(-2);
+              const FT xs = Bt * (FT)(dn ? 2 * nb2 : -2 * nb2);
+              uint64_t flips = 0;
               while (true) {
-                const FT fk = Ft.dyn(k);
-                const bool dn = (sb >> lane) & 1ull;
-                const FT dEcl = dn ? (FT)2 * fk : (FT)(-2) * fk;
-                const FT dEtr = (FT)(dn ? 2 * nb2 : -2 * nb2);
-                const FT x = Bt * dEtr;
-                const FT dE = fma_ft<FT>(at, dEcl, x);
+                FT dE;
+                if constexpr (PPW < 8) {
+                  dE = fma_ft<FT>(at, s2 * Ft.dyn(k), xs);
+                } else {
+                  const FT fk = Ft.dyn(k);
+                  const bool dl = (sb >> lane) & 1ull;
+                  const FT dEcl = dl ? (FT)2 * fk : (FT)(-2) * fk;
+                  const FT dEtr = (FT)(dl ? 2 * nb2 : -2 * nb2);
+                  dE = fma_ft<FT>(at, dEcl, Bt * dEtr);
+                }
                 const uint64_t cand = ballot64(dE < th) & live;
                 if (cand == 0) break;
                 const int i0 = __builtin_ctzll(cand);
                 live &= lanes_above(i0);
                 const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
                 sb ^= 1ull << i0;
-                if (lane == i0) dw ^= 1ull << k;
+                flips |= 1ull << i0;
                 row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
                 ++nflip;
               }
+              dw ^= ((flips >> lane) & 1ull) << k;
               if (lane == 0) s_down[t * NS + k] = sb;
             }
           }
