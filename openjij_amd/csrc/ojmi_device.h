@@ -130,6 +130,17 @@ __device__ __forceinline__ double sweep_scalar(double x) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// The value lane `i` (wave-uniform) holds, in scalar registers.
+__device__ __forceinline__ float lane_value(float x, int i) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), i));
+}
+__device__ __forceinline__ double lane_value(double x, int i) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, i);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), i);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 constexpr int RULE_METROPOLIS = 0;
 constexpr int RULE_HEAT_BATH = 1;
 template <typename TT>

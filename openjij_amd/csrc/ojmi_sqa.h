@@ -110,7 +110,7 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
               const FT s2 = dn ? (FT)2 : (FT)(-2);
               const FT xs = Bt * (FT)(dn ? 2 * nb2 : -2 * nb2);
               uint64_t flips = 0;
-              while (true) {
+              auto candidates = [&]() -> uint64_t {
                 FT dE;
                 if constexpr (PPW < 8) {
                   dE = fma_ft<FT>(at, s2 * Ft.dyn(k), xs);
@@ -121,16 +121,26 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
                   const FT dEtr = (FT)(dl ? 2 * nb2 : -2 * nb2);
                   dE = fma_ft<FT>(at, dEcl, Bt * dEtr);
                 }
-                const uint64_t cand = ballot64(dE < th) & live;
-                if (cand == 0) break;
+                return ballot64(dE < th) & live;
+              };
+              // The chain of one slot runs on the scalar unit, which the
+              // waves of a SIMD share: the lowest candidate and the lanes at
+              // or below it come from one subtraction (below = bits 0..i0),
+              // the flip's coefficient is the flipping lane's own s2 (its
+              // spin before the flip), the slot's spin word and flip count
+              // are formed once per slot.
+              uint64_t cand = candidates();
+              while (cand != 0) {
+                const uint64_t below = cand ^ (cand - 1);
                 const int i0 = __builtin_ctzll(cand);
-                live &= lanes_above(i0);
-                const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
-                sb ^= 1ull << i0;
-                flips |= 1ull << i0;
+                live &= ~below;
+                flips |= cand & below;
+                const FT c = lane_value(s2, i0);
                 row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
-                ++nflip;
+                cand = candidates();
               }
+              sb ^= flips;
+              nflip += (uint64_t)__builtin_popcountll(flips);
               dw ^= ((flips >> lane) & 1ull) << k;
               if (lane == 0) s_down[t * NS + k] = sb;
             }
