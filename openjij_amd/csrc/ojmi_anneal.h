@@ -74,19 +74,39 @@ __device__ __forceinline__ uint64_t decide_slot(FieldVec<FT, NS>& F, const char*
   uint64_t live = ballot64(i < n);
   // a lane is never tested again once it has flipped, so every live
   // lane keeps its sign within the slot: dE = s2 * F
-  const uint64_t sb = ballot64((down >> k) & 1ull);
   const FT s2 = ((down >> k) & 1ull) ? (FT)2 : (FT)(-2);
   uint64_t flips = 0;
-  while (true) {
-    const FT fk = F.dyn(k);
-    const FT dE = s2 * fk;
-    const uint64_t cand = ballot64((TT)dE < th) & live;
-    if (cand == 0) break;
-    const int i0 = __builtin_ctzll(cand);
-    live &= lanes_above(i0);
-    const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
-    flips |= 1ull << i0;
-    row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c, ns);
+  // (the form below takes 16 more registers with 16 or 32 slots of INT32
+  // fields, which costs the batch kernel two waves per SIMD, and moves 64
+  // slots of fields to scratch; those shapes keep the second form)
+  constexpr bool kShortChain = NS <= 8 || (NS <= 32 && (FT)0.5 != (FT)0);
+  if constexpr (kShortChain) {
+    // The chain runs on the scalar unit, which the waves of a SIMD share:
+    // the lowest candidate and the lanes at or below it come from one
+    // subtraction (below = bits 0..i0), and the flip's coefficient is the
+    // flipping lane's own s2.
+    uint64_t cand = ballot64((TT)(s2 * F.dyn(k)) < th) & live;
+    while (cand != 0) {
+      const uint64_t below = cand ^ (cand - 1);
+      const int i0 = __builtin_ctzll(cand);
+      live &= ~below;
+      flips |= cand & below;
+      row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, lane_value(s2, i0), ns);
+      cand = ballot64((TT)(s2 * F.dyn(k)) < th) & live;
+    }
+  } else {
+    const uint64_t sb = ballot64((down >> k) & 1ull);
+    while (true) {
+      const FT fk = F.dyn(k);
+      const FT dE = s2 * fk;
+      const uint64_t cand = ballot64((TT)dE < th) & live;
+      if (cand == 0) break;
+      const int i0 = __builtin_ctzll(cand);
+      live &= lanes_above(i0);
+      const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
+      flips |= 1ull << i0;
+      row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c, ns);
+    }
   }
   down ^= ((flips >> lane) & 1ull) << k;
   return (uint64_t)__builtin_popcountll(flips);

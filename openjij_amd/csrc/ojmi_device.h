@@ -131,6 +131,9 @@ __device__ __forceinline__ double sweep_scalar(double x) {
 }
 
 // The value lane `i` (wave-uniform) holds, in scalar registers.
+__device__ __forceinline__ int32_t lane_value(int32_t x, int i) {
+  return __builtin_amdgcn_readlane(x, i);
+}
 __device__ __forceinline__ float lane_value(float x, int i) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), i));
 }
