@@ -313,14 +313,16 @@ void k_dense_mfma(SharedArgs a, const char* __restrict__ Jpk) {
         // within the block every live lane keeps its sign: dE = s2 * F
         const FT s2 = ((sb >> lane) & 1ull) ? (FT)2 : (FT)(-2);
         FT fk = sF[w * 64 + lane];
-        while (true) {
-          const uint64_t cand = ballot64((TT)(s2 * fk) < th) & live;
-          if (cand == 0ull) break;
-          const uint64_t low = cand & (~cand + 1ull);  // the next flip, i0
-          live &= ~(cand ^ (cand - 1ull));              // lanes <= i0 are done
-          flips |= low;
-          const FT c = coef2<FT>((sb & low) != 0ull);
-          fk = fma_ft<FT>(c, Dc.dyn(__builtin_ctzll(cand)), fk);
+        // the chain runs on the scalar unit: the next flip i0 and the lanes
+        // <= i0 (done) from one subtraction, the coefficient from lane i0
+        uint64_t cand = ballot64((TT)(s2 * fk) < th) & live;
+        while (cand != 0ull) {
+          const uint64_t below = cand ^ (cand - 1ull);
+          const int i0 = __builtin_ctzll(cand);
+          live &= ~below;
+          flips |= cand & below;
+          fk = fma_ft<FT>(lane_value(s2, i0), Dc.dyn(i0), fk);
+          cand = ballot64((TT)(s2 * fk) < th) & live;
         }
       }
       if (lane == 0) {
@@ -678,23 +680,26 @@ void k_dense_mfma_split(SharedArgs a, const char* __restrict__ Jpk) {
         const FT s21 = ((sb1 >> lane) & 1ull) ? (FT)2 : (FT)(-2);
         FT fk0 = sF[g0 * 64 + lane];
         FT fk1 = sF[g1 * 64 + lane];
+        // two chains interleaved, each on the scalar unit as in the uniform
+        // form (next flip and done lanes from one subtraction, coefficient
+        // from the flipping lane)
         while (true) {
           const uint64_t cand0 = ballot64((TT)(s20 * fk0) < th0) & live0;
           const uint64_t cand1 = ballot64((TT)(s21 * fk1) < th1) & live1;
           if ((cand0 | cand1) == 0ull) break;
           if (cand0) {
-            const uint64_t low = cand0 & (~cand0 + 1ull);
-            live0 &= ~(cand0 ^ (cand0 - 1ull));
-            flips0 |= low;
-            const FT c = coef2<FT>((sb0 & low) != 0ull);
-            fk0 = fma_ft<FT>(c, Dc.dyn(__builtin_ctzll(cand0)), fk0);
+            const uint64_t below = cand0 ^ (cand0 - 1ull);
+            const int i0 = __builtin_ctzll(cand0);
+            live0 &= ~below;
+            flips0 |= cand0 & below;
+            fk0 = fma_ft<FT>(lane_value(s20, i0), Dc.dyn(i0), fk0);
           }
           if (cand1) {
-            const uint64_t low = cand1 & (~cand1 + 1ull);
-            live1 &= ~(cand1 ^ (cand1 - 1ull));
-            flips1 |= low;
-            const FT c = coef2<FT>((sb1 & low) != 0ull);
-            fk1 = fma_ft<FT>(c, Dc.dyn(__builtin_ctzll(cand1)), fk1);
+            const uint64_t below = cand1 ^ (cand1 - 1ull);
+            const int i1 = __builtin_ctzll(cand1);
+            live1 &= ~below;
+            flips1 |= cand1 & below;
+            fk1 = fma_ft<FT>(lane_value(s21, i1), Dc.dyn(i1), fk1);
           }
         }
       }

@@ -347,16 +347,16 @@ void k_dense_colown(SharedArgs a) {
           sb0 = sb;
           const FT s2 = dn ? (FT)2 : (FT)(-2);
           FT fk = sF[w * 64 + lane];
-          while (true) {
-            const FT dE = s2 * fk;
-            const uint64_t cand = ballot64((TT)dE < th) & live;
-            if (cand == 0) break;
-            const uint64_t low = cand & (~cand + 1ull);  // the next flip, i0
-            live &= ~(cand ^ (cand - 1ull));              // lanes <= i0 are done
-            flips |= low;
+          // the chain runs on the scalar unit: the next flip i0 and the lanes
+          // <= i0 (done) from one subtraction, the coefficient from lane i0
+          uint64_t cand = ballot64((TT)(s2 * fk) < th) & live;
+          while (cand != 0) {
+            const uint64_t below = cand ^ (cand - 1ull);
             const int i0 = __builtin_ctzll(cand);
-            const FT c = coef2<FT>((sb & low) != 0ull);
-            fk = fma_ft<FT>(c, Dc.dyn(i0), fk);
+            live &= ~below;
+            flips |= cand & below;
+            fk = fma_ft<FT>(lane_value(s2, i0), Dc.dyn(i0), fk);
+            cand = ballot64((TT)(s2 * fk) < th) & live;
           }
         }
         if (lane == 0) {
