@@ -76,22 +76,31 @@ __device__ __forceinline__ uint64_t decide_slot(FieldVec<FT, NS>& F, const char*
   // lane keeps its sign within the slot: dE = s2 * F
   const FT s2 = ((down >> k) & 1ull) ? (FT)2 : (FT)(-2);
   uint64_t flips = 0;
-  // (the form below takes 16 more registers with 16 or 32 slots of INT32
-  // fields, which costs the batch kernel two waves per SIMD, and moves 64
-  // slots of fields to scratch; those shapes keep the second form)
-  constexpr bool kShortChain = NS <= 8 || (NS <= 32 && (FT)0.5 != (FT)0);
+  // (64 slots: the form below moves the fields to scratch; INT32 fields with
+  // int32 couplings and 8 or more slots: it takes up to 30 more registers;
+  // those keep the second form)
+  constexpr bool kReal = (FT)0.5 != (FT)0;
+  constexpr bool kShortChain = NS <= 32 && (kReal || sizeof(JT) == 1 || NS < 8);
   if constexpr (kShortChain) {
     // The chain runs on the scalar unit, which the waves of a SIMD share:
     // the lowest candidate and the lanes at or below it come from one
-    // subtraction (below = bits 0..i0), and the flip's coefficient is the
-    // flipping lane's own s2.
+    // subtraction (below = bits 0..i0).  The flip's coefficient is the
+    // flipping lane's own s2, read from that lane for real fields; integer
+    // fields take it as a select of +-2 from the sign word, which keeps the
+    // row update a 32-bit multiply-add (a read-lane value is an arbitrary
+    // integer to the compiler, and the update became 64-bit).
+    uint64_t sb = 0;
+    if constexpr (!kReal) sb = ballot64((down >> k) & 1ull);
     uint64_t cand = ballot64((TT)(s2 * F.dyn(k)) < th) & live;
     while (cand != 0) {
       const uint64_t below = cand ^ (cand - 1);
       const int i0 = __builtin_ctzll(cand);
       live &= ~below;
       flips |= cand & below;
-      row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, lane_value(s2, i0), ns);
+      FT c;
+      if constexpr (kReal) c = lane_value(s2, i0);
+      else c = (sb & cand & below) ? (FT)2 : (FT)(-2);
+      row_axpy<JT, FT, NS>(F, J + (int64_t)(64 * k + i0) * RB, lane, c, ns);
       cand = ballot64((TT)(s2 * F.dyn(k)) < th) & live;
     }
   } else {
