@@ -22,14 +22,24 @@
 
 namespace ojmi {
 
-template <typename FT, int DMAX>
+// LDS bytes of the slices' spins, rounded up to 16 for the word regions after them
+__host__ __device__ inline size_t sqa_csr_spin_bytes(int n, int M) {
+  return ((size_t)n * (size_t)M + 15) & ~(size_t)15;
+}
+
+// STAGE: each wave writes the Philox words of its slice's attempts to its own
+// LDS region before the slice's passes (one Philox evaluation per 4 spins
+// instead of one per spin); else every attempt evaluates its own.
+template <typename FT, int DMAX, bool STAGE>
 __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
-  extern __shared__ int8_t s_spin[];  // [M][n]
+  extern __shared__ int8_t s_spin[];  // [M][n], then (STAGE) words [W][4 ng]
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int W = blockDim.x >> 6;
   const int r = blockIdx.x;
   const int n = a.n, M = a.M;
+  const int ng = (n + 3) >> 2;
+  uint32_t* s_w = reinterpret_cast<uint32_t*>(s_spin + sqa_csr_spin_bytes(n, M)) + (size_t)w * 4 * ng;
   int8_t* sp = a.spins + (int64_t)r * M * n;
   for (int e = threadIdx.x; e < M * n; e += blockDim.x) s_spin[e] = sp[e] < 0 ? (int8_t)-1 : (int8_t)1;
   __syncthreads();
@@ -61,6 +71,16 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
         int8_t* s = s_spin + (int64_t)t * n;
         const int8_t* su = s_spin + (int64_t)((t + 1) % M) * n;
         const int8_t* sd = s_spin + (int64_t)((t + M - 1) % M) * n;
+        if constexpr (STAGE) {
+          for (int g = lane; g < ng; g += 64) {
+            uint32_t w0, w1, w2, w3;
+            philox4x32_10((uint32_t)g, tg, rep, (uint32_t)t, slo, shi, w0, w1, w2, w3);
+            s_w[4 * g + 0] = w0;
+            s_w[4 * g + 1] = w1;
+            s_w[4 * g + 2] = w2;
+            s_w[4 * g + 3] = w3;
+          }
+        }
         for (int lv = 0; lv < a.nlev; ++lv) {
           const int b0 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv]);
           const int b1 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv + 1]);
@@ -83,9 +103,14 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
               const FT dEtr = (FT)(-2 * si * ((int)su[i] + (int)sd[i]));
               const FT x = Bt * dEtr;
               const FT dE = fma_ft<FT>(at, dEcl, x);
-              uint32_t u0, u1, u2, u3;
-              philox4x32_10((uint32_t)(i >> 2), tg, rep, (uint32_t)t, slo, shi, u0, u1, u2, u3);
-              const uint32_t u = (i & 2) ? ((i & 1) ? u3 : u2) : ((i & 1) ? u1 : u0);
+              uint32_t u;
+              if constexpr (STAGE) {
+                u = s_w[i];
+              } else {
+                uint32_t u0, u1, u2, u3;
+                philox4x32_10((uint32_t)(i >> 2), tg, rep, (uint32_t)t, slo, shi, u0, u1, u2, u3);
+                u = (i & 2) ? ((i & 1) ? u3 : u2) : ((i & 1) ? u1 : u0);
+              }
               flipped = dE < -log_uniform<FT>(u);
               if (flipped) s[i] = (int8_t)-si;
             }
@@ -102,22 +127,37 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
 
 size_t sqa_csr_lds_bytes(int n, int M) { return (size_t)n * (size_t)M; }
 
+int sqa_csr_waves(int M) {
+  const int P = (M + 1) / 2;
+  return P < 16 ? P : 16;
+}
+
+size_t sqa_csr_staged_lds_bytes(int n, int M) {
+  return sqa_csr_spin_bytes(n, M) + (size_t)sqa_csr_waves(M) * 16 * (size_t)((n + 3) / 4);
+}
+
+template <typename FT, int DMAX>
+static hipError_t launch_sqa_csr_d(const CsrArgs& a, bool stage, dim3 grid, dim3 block,
+                                   hipStream_t st) {
+  const size_t lds = stage ? sqa_csr_staged_lds_bytes(a.n, a.M) : sqa_csr_lds_bytes(a.n, a.M);
+  auto kern = stage ? k_sqa_csr<FT, DMAX, true> : k_sqa_csr<FT, DMAX, false>;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, grid, block, lds, st, a);
+  return hipSuccess;
+}
+
 template <typename FT>
-static hipError_t launch_sqa_csr_ft(const CsrArgs& a, hipStream_t st) {
-  const int P = (a.M + 1) / 2;
-  const int W = P < 16 ? P : 16;
-  const size_t lds = sqa_csr_lds_bytes(a.n, a.M);
+static hipError_t launch_sqa_csr_ft(const CsrArgs& a, bool stage, hipStream_t st) {
+  const int W = sqa_csr_waves(a.M);
   dim3 grid(a.R), block(64 * W);
-#define OJMI_CASE(dm)                                                                   \
-  case dm: {                                                                            \
-    auto kern = k_sqa_csr<FT, dm>;                                                      \
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),             \
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,      \
-                                       (int)lds);                                       \
-    if (e != hipSuccess) return e;                                                      \
-    hipLaunchKernelGGL(kern, grid, block, lds, st, a);                                  \
-    break;                                                                              \
-  }
+  hipError_t e = hipSuccess;
+#define OJMI_CASE(dm)                                              \
+  case dm:                                                         \
+    e = launch_sqa_csr_d<FT, dm>(a, stage, grid, block, st);       \
+    if (e != hipSuccess) return e;                                 \
+    break;
   switch (a.D) {
     OJMI_CASE(8)
     OJMI_CASE(16)
@@ -130,11 +170,11 @@ static hipError_t launch_sqa_csr_ft(const CsrArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_sqa_csr(int ftype, const CsrArgs& a, hipStream_t st) {
+hipError_t launch_sqa_csr(int ftype, const CsrArgs& a, bool stage, hipStream_t st) {
   if (a.R <= 0 || a.T <= 0) return hipSuccess;
   if (a.M < 2) return hipErrorInvalidValue;
-  if (ftype == FT_F64) return launch_sqa_csr_ft<double>(a, st);
-  if (ftype == FT_F32) return launch_sqa_csr_ft<float>(a, st);
+  if (ftype == FT_F64) return launch_sqa_csr_ft<double>(a, stage, st);
+  if (ftype == FT_F32) return launch_sqa_csr_ft<float>(a, stage, st);
   return hipErrorInvalidValue;
 }
 

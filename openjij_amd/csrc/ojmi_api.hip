@@ -728,8 +728,21 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.p1_f = c->dp1_f + t_begin;
     a.rule = RULE_METROPOLIS;
     a.stats = c->dstats;
-    c->lds_used = (int)sqa_csr_lds_bytes((int)c->n, (int)c->M);
-    return launch_sqa_csr(c->ftype, a, c->st);
+    // The slices' Philox words are staged in LDS (one Philox per 4 spins
+    // instead of one per spin) where they fit beside the spins and leave as
+    // many workgroups resident per CU as the replicas can use (at most 28
+    // waves per CU at this kernel's register count).
+    const size_t l0 = sqa_csr_lds_bytes((int)c->n, (int)c->M);
+    const size_t l1 = sqa_csr_staged_lds_bytes((int)c->n, (int)c->M);
+    const int64_t need = (c->R + c->num_cus - 1) / c->num_cus;
+    auto resident = [&](size_t lds) -> int64_t {
+      return std::min<int64_t>((int64_t)((size_t)c->lds_budget / std::max<size_t>(lds, 1)),
+                               28 / sqa_csr_waves((int)c->M));
+    };
+    const bool stage = l1 <= (size_t)c->lds_budget && resident(l1) >= std::min(need, resident(l0));
+    c->lds_used = (int)(stage ? sqa_csr_staged_lds_bytes((int)c->n, (int)c->M)
+                              : sqa_csr_lds_bytes((int)c->n, (int)c->M));
+    return launch_sqa_csr(c->ftype, a, stage, c->st);
   }
   void* F_out = carries_fields(c) ? c->dF : nullptr;
   DenseArgs a{};

@@ -163,8 +163,12 @@ hipError_t launch_csr_level(int ftype, const CsrArgs& a, int waves_per_block, in
 
 // Trotter sweeps of CSR systems (k_sqa_csr.hip): one workgroup per replica,
 // spins of the M slices in LDS (n * M bytes), a.M >= 2, a.D the ELL width.
+// stage: each wave also holds its slice's Philox words in LDS
+// (sqa_csr_staged_lds_bytes in total).
 size_t sqa_csr_lds_bytes(int n, int M);
-hipError_t launch_sqa_csr(int ftype, const CsrArgs& a, hipStream_t st);
+size_t sqa_csr_staged_lds_bytes(int n, int M);
+int sqa_csr_waves(int M);  // waves per workgroup (one slice of a phase each)
+hipError_t launch_sqa_csr(int ftype, const CsrArgs& a, bool stage, hipStream_t st);
 
 // Automatic-schedule statistics of every problem of a batch (k_beta_stats.hip):
 // out[k] = {dE_max, dE_min, lin_max, mean_abs, int_min} from the spin pair

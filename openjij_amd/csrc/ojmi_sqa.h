@@ -129,15 +129,36 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
               // the flip's coefficient is the flipping lane's own s2 (its
               // spin before the flip), the slot's spin word and flip count
               // are formed once per slot.
-              uint64_t cand = candidates();
-              while (cand != 0) {
-                const uint64_t below = cand ^ (cand - 1);
-                const int i0 = __builtin_ctzll(cand);
-                live &= ~below;
-                flips |= cand & below;
-                const FT c = lane_value(s2, i0);
-                row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
-                cand = candidates();
+              // (64 slots: this form moves the fields to scratch, so those
+              // slices keep the per-flip updates of sb, dw and the count.)
+              if constexpr (NS <= 32) {
+                uint64_t cand = candidates();
+                while (cand != 0) {
+                  const uint64_t below = cand ^ (cand - 1);
+                  const int i0 = __builtin_ctzll(cand);
+                  live &= ~below;
+                  flips |= cand & below;
+                  const FT c = lane_value(s2, i0);
+                  row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
+                  cand = candidates();
+                }
+              } else {
+                while (true) {
+                  const FT fk = Ft.dyn(k);
+                  const bool dl = (sb >> lane) & 1ull;
+                  const FT dEcl = dl ? (FT)2 * fk : (FT)(-2) * fk;
+                  const FT dEtr = (FT)(dl ? 2 * nb2 : -2 * nb2);
+                  const FT dE = fma_ft<FT>(at, dEcl, Bt * dEtr);
+                  const uint64_t cand = ballot64(dE < th) & live;
+                  if (cand == 0) break;
+                  const int i0 = __builtin_ctzll(cand);
+                  live &= lanes_above(i0);
+                  const FT c = ((sb >> i0) & 1ull) ? (FT)2 : (FT)(-2);
+                  sb ^= 1ull << i0;
+                  if (lane == i0) dw ^= 1ull << k;
+                  row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
+                  ++nflip;
+                }
               }
               sb ^= flips;
               nflip += (uint64_t)__builtin_popcountll(flips);

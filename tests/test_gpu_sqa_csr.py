@@ -86,7 +86,9 @@ def test_sqa_csr_bitexact(kind, size, M, prec, R, L):
     want = orc.replay_sqa_csr(prec, rp, ci, va, n, init, 0x7777, 5, a, B)
     np.testing.assert_array_equal(got, want)
     assert info["last_attempts"] == L * R * M * n
-    assert info["lds_bytes"] == n * M
+    # spins of the slices, then (when they fit) each wave's Philox words
+    staged = ((n * M + 15) // 16) * 16 + min(16, (M + 1) // 2) * 16 * ((n + 3) // 4)
+    assert info["lds_bytes"] in (n * M, staged)
     check_energies(rp, ci, va, n, got, en.reshape(R, M))
     for r in range(R):
         assert list(sol[r]) == list(got[r, int(np.argmin(en.reshape(R, M)[r]))])
