@@ -192,6 +192,12 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *                    rows from there, 0 reads them from L2
  *   "lds_rows_wpb":  replicas per workgroup of that form (0 = by R: 1 up to
  *                    256 replicas, then up to 4)
+ *   "sqa_rows_lds":  dense Trotter contexts of <= 512 spins: 1 (default)
+ *                    keeps the first 1, 2 or 4 slots (64 columns each) of
+ *                    every coupling row in LDS, as many as fit beside the
+ *                    slices' spins (2 of 4 at 256 FP32 spins), and reads only
+ *                    the other slots of a flipped row from L2; 0 reads whole
+ *                    rows from L2; results are identical
  *   "wave_helpers":  one-wave-per-replica dense contexts of n <= 512: helper
  *                    waves (1..3) per replica that compute its thresholds one
  *                    phase of 4 sweeps ahead, 0 = none, -1 (default) = 2 while

@@ -16,4 +16,17 @@ hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W
   return hipErrorInvalidValue;
 }
 
+int sqa_lds_slots(int ftype, int NS, int n, int M, size_t budget) {
+  // the widest of 4, 2, 1 slots per row whose copy (built where the largest
+  // system of NS slots fits 160 KiB) fits beside the slices' spin words
+  if (ftype != FT_F32 && ftype != FT_F64) return 0;
+  const size_t fb = ftype == FT_F64 ? 8 : 4;
+  const size_t spin_words = (size_t)M * (size_t)NS * 8;
+  for (int sl : {4, 2, 1}) {
+    const bool built = sl <= NS && (size_t)sl * 64 * NS * 64 * fb <= 160 * 1024;
+    if (built && spin_words + (size_t)sl * (size_t)n * 64 * fb <= budget) return sl;
+  }
+  return 0;
+}
+
 }  // namespace ojmi

@@ -12,11 +12,11 @@ hipError_t launch_sqa_wide(int ftype, const DenseArgs& a, int PPW, int W, hipStr
   dim3 grid(a.R), block(64 * W);
   const size_t lds = (size_t)a.M * 64 * sizeof(uint64_t);
   if (ftype == FT_F64 && PPW == 1) {
-    hipLaunchKernelGGL((k_sqa_wave<double, 64, 1, 256>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((k_sqa_wave<double, 64, 1, 256, 0>), grid, block, lds, st, a);
   } else if (ftype == FT_F32 && PPW == 1) {
-    hipLaunchKernelGGL((k_sqa_wave<float, 64, 1, 256>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((k_sqa_wave<float, 64, 1, 256, 0>), grid, block, lds, st, a);
   } else if (ftype == FT_F32 && PPW == 2) {
-    hipLaunchKernelGGL((k_sqa_wave<float, 64, 2, 256>), grid, block, lds, st, a);
+    hipLaunchKernelGGL((k_sqa_wave<float, 64, 2, 256, 0>), grid, block, lds, st, a);
   } else {
     return hipErrorInvalidValue;
   }

@@ -94,6 +94,7 @@ struct ojmi_ctx {
   int mfma_sweep = 0;
   int mfma_accept = 85;  // per mille: the matrix-core sweep at or above this rate
   int mfma_form = 1;     // 1: four waves update the fields while four decide; 0: all do both
+  int sqa_rows_lds = 1;  // Trotter kernel: first slots of every coupling row in LDS (0: none)
   void* dJpk = nullptr;  // packed MFMA operands of the couplings (k_dense_mfma)
   int32_t* dsel = nullptr;  // segment kernel choice and flip snapshot (mode 2)
   int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
@@ -766,6 +767,21 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
   a.stats = c->dstats;
   if (c->family == OJMI_KERNEL_SQA_WAVE) {
     a.F_in = a.F_out = nullptr;
+    // The first slots of every coupling row are read from LDS (copied once
+    // per workgroup) and the others from L2: a flip's row then costs the
+    // vector-memory return path, which binds this kernel, only its other
+    // slots.  The workgroups of one CU share its LDS when the replicas
+    // outnumber the CUs.
+    const size_t spin_words = (size_t)c->M * (size_t)c->NS * 8;
+    const size_t fb = c->ftype == FT_F64 ? 8 : 4;
+    int kc = 0;
+    if (c->sqa_rows_lds && c->NS <= 8) {
+      const int64_t per_cu = std::min<int64_t>(2, (c->R + c->num_cus - 1) / c->num_cus);
+      kc = sqa_lds_slots(c->ftype, (int)c->NS, (int)c->n, (int)c->M,
+                         (size_t)c->lds_budget / (size_t)std::max<int64_t>(1, per_cu));
+    }
+    a.cache_rows = kc;
+    c->lds_used = (int)(spin_words + (size_t)kc * (size_t)c->n * 64 * fb);
     return launch_sqa_wave(c->ftype, a, c->NS, c->sqa_ppw, c->sqa_w, c->st);
   }
   if (uses_mfma_sweep(c)) {
@@ -1522,6 +1538,11 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   if (k == "mfma_form") {
     if (value != 0 && value != 1) return fail(OJMI_EINVAL, "mfma_form must be 0 or 1");
     c->mfma_form = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "sqa_rows_lds") {
+    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "sqa_rows_lds must be 0 or 1");
+    c->sqa_rows_lds = (int)value;
     return OJMI_OK;
   }
   if (k == "mfma_accept") {

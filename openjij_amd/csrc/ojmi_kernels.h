@@ -27,6 +27,7 @@ struct DenseArgs {
   void* F_out;         // optional [R][Npad] final fields
   int rule;            // classical acceptance rule: RULE_METROPOLIS / RULE_HEAT_BATH
   unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
+  int cache_rows = 0;  // Trotter kernel: slots of every coupling row held in LDS
 };
 
 // Batch of independent dense classical problems (k_batch.hip, k_batch_wave):
@@ -150,6 +151,9 @@ hipError_t launch_dense_lds(int ftype, int jtype, const DenseArgs& a, int Npad, 
 hipError_t launch_batch_energy(int ftype, int jtype, const BatchArgs& b, const int64_t* rb, int K,
                                double* E, hipStream_t st);
 hipError_t launch_sqa_wave(int ftype, const DenseArgs& a, int NS, int PPW, int W, hipStream_t st);
+// Trotter kernel: a.cache_rows = slots (of 64 columns) of every coupling row
+// held in LDS, as chosen by sqa_lds_slots for an LDS budget per workgroup.
+int sqa_lds_slots(int ftype, int NS, int n, int M, size_t budget);
 // ELL width of the CSR kernels for maximum degree D (8, 16, 32 or 64).
 int csr_padded_width(int D);
 // Waves per replica the CSR launcher uses (1 = one wave per replica, several

@@ -16,10 +16,10 @@ namespace ojmi {
 // lane can read the up/down neighbours of its own spin.
 //   dEcl = -2 s F,  dEtr = -2 s (s_up + s_dn),  dE = fma(a, dEcl, B*dEtr),
 //   flip iff dE < -log(U)  (a = s*beta/M, B = 1/2 log tanh(beta*Gamma*(1-s)/M)).
-template <typename FT, int NS, int PPW, int MAXT>
+template <typename FT, int NS, int PPW, int MAXT, int SL>
 __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
   typedef FT TT;
-  extern __shared__ uint64_t s_down[];  // [M][NS]
+  extern __shared__ uint64_t s_down[];  // [M][NS], then (SL > 0) slots [SL][n][64] of every row
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int W = blockDim.x >> 6;
@@ -29,6 +29,16 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
   const char* J = reinterpret_cast<const char*>(a.J);
   constexpr int64_t RB = row_bytes<FT, NS>();
   const FT* h = reinterpret_cast<const FT*>(a.h);
+
+  // SL > 0: slots 0..SL-1 of every coupling row copied to LDS once
+  // ([slot][row][64]); a flip reads those slots there and the others from L2
+  FT* sJ = reinterpret_cast<FT*>(s_down + M * NS);
+  if constexpr (SL > 0) {
+    for (int e = threadIdx.x; e < SL * n * 64; e += blockDim.x) {
+      const int m = e / (n * 64), rl = e - m * n * 64, row = rl >> 6, l = rl & 63;
+      sJ[e] = reinterpret_cast<const FT*>(J + (int64_t)row * RB)[m * 64 + l];
+    }
+  }
 
   uint64_t down[2 * PPW];
   FieldVec<FT, NS> F[2 * PPW];
@@ -129,9 +139,10 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
               // the flip's coefficient is the flipping lane's own s2 (its
               // spin before the flip), the slot's spin word and flip count
               // are formed once per slot.
-              // (64 slots: this form moves the fields to scratch, so those
-              // slices keep the per-flip updates of sb, dw and the count.)
-              if constexpr (NS <= 32) {
+              // (64 slots, or 8 slice pairs per wave: this form moves the
+              // fields to scratch, so those keep the per-flip updates of sb,
+              // dw and the count.)
+              if constexpr (NS <= 32 && PPW < 8) {
                 uint64_t cand = candidates();
                 while (cand != 0) {
                   const uint64_t below = cand ^ (cand - 1);
@@ -139,7 +150,18 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
                   live &= ~below;
                   flips |= cand & below;
                   const FT c = lane_value(s2, i0);
-                  row_axpy<FT, FT, NS>(Ft, J + (int64_t)(64 * k + i0) * RB, lane, c);
+                  const int row = 64 * k + i0;
+                  if constexpr (SL > 0) {
+                    const FT* pg = reinterpret_cast<const FT*>(J + (int64_t)row * RB);
+                    FT x[NS];
+#pragma unroll
+                    for (int m = 0; m < NS; ++m)
+                      x[m] = (m < SL) ? sJ[(m * n + row) * 64 + lane] : pg[m * 64 + lane];
+#pragma unroll
+                    for (int m = 0; m < NS; ++m) Ft.set(m, fma_ft<FT>(c, x[m], Ft.get(m)));
+                  } else {
+                    row_axpy<FT, FT, NS>(Ft, J + (int64_t)row * RB, lane, c);
+                  }
                   cand = candidates();
                 }
               } else {
@@ -194,41 +216,60 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
   }
 }
 
-template <typename FT, int NS>
-inline hipError_t launch_sqa_ns(const DenseArgs& a, int PPW, int W, hipStream_t st) {
-  dim3 grid(a.R), block(64 * W);
-  const size_t lds = (size_t)a.M * NS * sizeof(uint64_t);
-  // W > 4 runs the 1024-thread variant (<= 128 VGPRs), W <= 4 the 256-thread one.
+template <typename FT, int NS, int PPW, int MAXT, int SL>
+inline hipError_t launch_sqa_one(const DenseArgs& a, int W, size_t lds, hipStream_t st) {
+  auto kern = k_sqa_wave<FT, NS, PPW, MAXT, SL>;
+  if constexpr (SL > 0) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(kern, dim3(a.R), dim3(64 * W), lds, st, a);
+  return hipGetLastError();
+}
+
+// W > 4 runs the 1024-thread variant (<= 128 VGPRs), W <= 4 the 256-thread one.
+template <typename FT, int NS, int SL>
+inline hipError_t launch_sqa_form(const DenseArgs& a, int PPW, int W, hipStream_t st) {
+  const size_t lds = (size_t)a.M * NS * sizeof(uint64_t) + (size_t)SL * a.n * 64 * sizeof(FT);
   if (W > 4) {
     switch (PPW) {
-      case 1:
-        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 1, 1024>), grid, block, lds, st, a);
-        break;
-      case 2:
-        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 2, 1024>), grid, block, lds, st, a);
-        break;
-      default:
-        return hipErrorInvalidValue;
-    }
-  } else {
-    switch (PPW) {
-      case 1:
-        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 1, 256>), grid, block, lds, st, a);
-        break;
-      case 2:
-        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 2, 256>), grid, block, lds, st, a);
-        break;
-      case 4:
-        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 4, 256>), grid, block, lds, st, a);
-        break;
-      case 8:
-        hipLaunchKernelGGL((k_sqa_wave<FT, NS, 8, 256>), grid, block, lds, st, a);
-        break;
-      default:
-        return hipErrorInvalidValue;
+      case 1: return launch_sqa_one<FT, NS, 1, 1024, SL>(a, W, lds, st);
+      case 2: return launch_sqa_one<FT, NS, 2, 1024, SL>(a, W, lds, st);
+      default: return hipErrorInvalidValue;
     }
   }
-  return hipGetLastError();
+  switch (PPW) {
+    case 1: return launch_sqa_one<FT, NS, 1, 256, SL>(a, W, lds, st);
+    case 2: return launch_sqa_one<FT, NS, 2, 256, SL>(a, W, lds, st);
+    case 4: return launch_sqa_one<FT, NS, 4, 256, SL>(a, W, lds, st);
+    case 8: return launch_sqa_one<FT, NS, 8, 256, SL>(a, W, lds, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// Slots of every row that the LDS can hold for the largest system of NS
+// slots (64 NS rows): built only where they fit 160 KiB.
+template <typename FT, int NS, int SL>
+constexpr bool sqa_slots_fit() {
+  return SL <= NS && (size_t)SL * 64 * NS * 64 * sizeof(FT) <= 160 * 1024;
+}
+
+// a.cache_rows: slots of every coupling row held in LDS (0, 1, 2 or 4)
+template <typename FT, int NS>
+inline hipError_t launch_sqa_ns(const DenseArgs& a, int PPW, int W, hipStream_t st) {
+  const int sl = a.cache_rows;
+  if (sl == 0) return launch_sqa_form<FT, NS, 0>(a, PPW, W, st);
+  if constexpr (sqa_slots_fit<FT, NS, 1>()) {
+    if (sl == 1) return launch_sqa_form<FT, NS, 1>(a, PPW, W, st);
+  }
+  if constexpr (sqa_slots_fit<FT, NS, 2>()) {
+    if (sl == 2) return launch_sqa_form<FT, NS, 2>(a, PPW, W, st);
+  }
+  if constexpr (sqa_slots_fit<FT, NS, 4>()) {
+    if (sl == 4) return launch_sqa_form<FT, NS, 4>(a, PPW, W, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 // launch_sqa_<ft>_<range>: NS in {1, 2, 4, 8} (small) or {16, 32} (large)

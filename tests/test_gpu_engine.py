@@ -355,9 +355,25 @@ def test_csr_dense_like_graph_runs_on_dense_kernel():
 
 
 # --- Trotter ------------------------------------------------------------------
+def sqa_lds_slots(n, M, prec, budget):
+    """Slots of every coupling row the dense Trotter kernel keeps in LDS."""
+    NS = 1 << max(0, int(np.ceil(np.log2((n + 63) // 64))))
+    fb = 8 if prec == "fp64" else 4
+    for sl in (4, 2, 1):
+        if sl <= NS and NS <= 8 and sl * 64 * NS * 64 * fb <= 160 * 1024 and \
+                M * NS * 8 + sl * n * 64 * fb <= budget:
+            return NS, sl
+    return NS, 0
+
+
 @pytest.mark.parametrize("n,M,prec", [(32, 2, "fp64"), (32, 3, "fp64"), (40, 4, "fp32"),
-                                      (70, 5, "fp64"), (64, 8, "fp32"), (130, 33, "fp32")])
-def test_sqa_dense_bitexact(n, M, prec):
+                                      (70, 5, "fp64"), (64, 8, "fp32"), (130, 33, "fp32"),
+                                      (256, 4, "fp32"), (200, 7, "fp32"), (256, 6, "fp64"),
+                                      (500, 5, "fp32")])
+@pytest.mark.parametrize("rows_lds", [1, 0])
+def test_sqa_dense_bitexact(n, M, prec, rows_lds):
+    """rows_lds 1 (default): the first slots of every coupling row read from
+    LDS, the others from L2; 0: whole rows from L2."""
     E = H.sk_gauss(n, 5000 + n + M, field=True)
     L = 20
     s = np.linspace(0.0, 1.0, L)
@@ -368,11 +384,16 @@ def test_sqa_dense_bitexact(n, M, prec):
     init = np.repeat(H.random_pm1(R, n, M)[:, None, :], M, axis=1)
     with capi.Context.dense(E, R, M, gamma, PREC[prec]) as ctx:
         assert ctx.info()["kernel"] == capi.KERNEL_SQA_WAVE
+        ctx.set_option("sqa_rows_lds", rows_lds)
         ctx.set_spins(init)
         ctx.run(beta, mcs, 1234, 0, s=s)
         got = ctx.get_spins()
         sol = ctx.solution()
         en = ctx.energies()
+        info = ctx.info()
+    NS, sl = sqa_lds_slots(n, M, prec, info["lds_budget"])
+    fb = 8 if prec == "fp64" else 4
+    assert info["lds_bytes"] == M * NS * 8 + (sl * n * 64 * fb if rows_lds else 0)
     a, B = orc.expand_transverse(beta, s, mcs, gamma, M)
     want = orc.replay_sqa_dense(prec, H.symmetric_ext(E), init, 1234, 0, a, B)
     np.testing.assert_array_equal(got, want)
@@ -429,8 +450,9 @@ def test_sqa_config_c_shape_replay():
     mcs = np.ones(L, dtype=np.int64)
     init = np.repeat(H.random_pm1(R, n, 3)[:, None, :], M, axis=1)
     a, B = orc.expand_transverse(beta, s, mcs, 5.0, M)
-    for prec in ("fp32", "fp64"):
+    for prec, rows_lds in (("fp32", 1), ("fp32", 0), ("fp64", 1)):
         with capi.Context.dense(E, R, M, 5.0, PREC[prec]) as ctx:
+            ctx.set_option("sqa_rows_lds", rows_lds)
             ctx.set_spins(init)
             ctx.run(beta, mcs, 0xC003, 0, s=s)
             got = ctx.get_spins()
