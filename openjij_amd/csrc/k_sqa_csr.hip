@@ -18,6 +18,7 @@
 // the level-order ELL arrays; U comes from the Philox word of (spin, sweep,
 // replica, slice), as in the dense Trotter kernel.
 #include "ojmi_kernels.h"
+#include "ojmi_csr.h"
 #include "ojmi_wave.h"
 
 namespace ojmi {
@@ -81,40 +82,75 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
             s_w[4 * g + 3] = w3;
           }
         }
-        for (int lv = 0; lv < a.nlev; ++lv) {
-          const int b0 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv]);
-          const int b1 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv + 1]);
-          for (int base = b0; base < b1; base += 64) {
-            const int pos = base + lane;
+        // one attempt: field from the neighbour list (stored order), the
+        // Trotter term from the adjacent slices, the Philox word's threshold
+        auto attempt = [&](int i, FT f) -> bool {
+          const int si = s[i];
+          const FT dEcl = (FT)(-2 * si) * f;
+          const FT dEtr = (FT)(-2 * si * ((int)su[i] + (int)sd[i]));
+          const FT x = Bt * dEtr;
+          const FT dE = fma_ft<FT>(at, dEcl, x);
+          uint32_t u;
+          if constexpr (STAGE) {
+            u = s_w[i];
+          } else {
+            uint32_t u0, u1, u2, u3;
+            philox4x32_10((uint32_t)(i >> 2), tg, rep, (uint32_t)t, slo, shi, u0, u1, u2, u3);
+            u = (i & 2) ? ((i & 1) ? u3 : u2) : ((i & 1) ? u1 : u0);
+          }
+          const bool fl = dE < -log_uniform<FT>(u);
+          if (fl) s[i] = (int8_t)-si;
+          return fl;
+        };
+        if constexpr (DMAX <= 8) {
+          // the next pass's neighbour lists in flight while this one runs
+          int lv = 0, base = 0;
+          CsrPass<FT, DMAX> cur;
+          csr_load_pass<FT, DMAX>(a, a.lvl_ptr, 0, 0, lane, cur);
+          while (true) {
+            const int b1 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv + 1]);
+            int nlv = lv, nbase = base + 64;
+            if (nbase >= b1) {
+              nlv = lv + 1;
+              nbase = b1;
+            }
+            const bool last = nlv >= a.nlev;
+            CsrPass<FT, DMAX> nxt;
+            if (!last) csr_load_pass<FT, DMAX>(a, a.lvl_ptr, nlv, nbase, lane, nxt);
             bool flipped = false;
-            if (pos < b1) {
-              const int i = a.order[pos];
-              FT f = hv[pos];
-              // eight neighbours in flight at a time (a full unroll of the
-              // wider lists would not fit the registers of 16 waves per CU)
-#pragma unroll 8
-              for (int d = 0; d < DMAX; ++d) {
-                const int col = a.ell_col[(int64_t)d * n + pos];
-                const FT v = ev[(int64_t)d * n + pos];
-                f = (s[col] < 0) ? f - v : f + v;
-              }
-              const int si = s[i];
-              const FT dEcl = (FT)(-2 * si) * f;
-              const FT dEtr = (FT)(-2 * si * ((int)su[i] + (int)sd[i]));
-              const FT x = Bt * dEtr;
-              const FT dE = fma_ft<FT>(at, dEcl, x);
-              uint32_t u;
-              if constexpr (STAGE) {
-                u = s_w[i];
-              } else {
-                uint32_t u0, u1, u2, u3;
-                philox4x32_10((uint32_t)(i >> 2), tg, rep, (uint32_t)t, slo, shi, u0, u1, u2, u3);
-                u = (i & 2) ? ((i & 1) ? u3 : u2) : ((i & 1) ? u1 : u0);
-              }
-              flipped = dE < -log_uniform<FT>(u);
-              if (flipped) s[i] = (int8_t)-si;
+            if (cur.valid) {
+              FT f = cur.hv;
+#pragma unroll
+              for (int d = 0; d < DMAX; ++d) f = (s[cur.col[d]] < 0) ? f - cur.val[d] : f + cur.val[d];
+              flipped = attempt(cur.i, f);
             }
             nflip += (uint64_t)__builtin_popcountll(ballot64(flipped));
+            if (last) break;
+            cur = nxt;
+            lv = nlv;
+            base = nbase;
+          }
+        } else {
+          for (int lv = 0; lv < a.nlev; ++lv) {
+            const int b0 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv]);
+            const int b1 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv + 1]);
+            for (int base = b0; base < b1; base += 64) {
+              const int pos = base + lane;
+              bool flipped = false;
+              if (pos < b1) {
+                FT f = hv[pos];
+                // eight neighbours in flight at a time (a full unroll of the
+                // wider lists would not fit the registers of 16 waves per CU)
+#pragma unroll 8
+                for (int d = 0; d < DMAX; ++d) {
+                  const int col = a.ell_col[(int64_t)d * n + pos];
+                  const FT v = ev[(int64_t)d * n + pos];
+                  f = (s[col] < 0) ? f - v : f + v;
+                }
+                flipped = attempt(a.order[pos], f);
+              }
+              nflip += (uint64_t)__builtin_popcountll(ballot64(flipped));
+            }
           }
         }
       }
