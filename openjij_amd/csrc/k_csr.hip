@@ -55,8 +55,9 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   const uint32_t rep = (uint32_t)(a.rep_offset + r);
   const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
 
+  const CsrBufs CB{};  // unused: plain global loads
   CsrPass<FT, DMAX> cur;
-  if constexpr (PREFETCH) csr_load_pass<FT, DMAX>(a, a.lvl_ptr, 0, 0, lane, cur);
+  if constexpr (PREFETCH) csr_load_pass<FT, DMAX, false>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
@@ -86,9 +87,9 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
       }
       CsrPass<FT, DMAX> nxt;
       if constexpr (PREFETCH) {
-        csr_load_pass<FT, DMAX>(a, a.lvl_ptr, nlv, nbase, lane, nxt);
+        csr_load_pass<FT, DMAX, false>(a, CB, a.lvl_ptr, nlv, nbase, lane, nxt);
       } else {
-        csr_load_pass<FT, DMAX>(a, a.lvl_ptr, lv, base, lane, cur);
+        csr_load_pass<FT, DMAX, false>(a, CB, a.lvl_ptr, lv, base, lane, cur);
       }
       bool flipped = false;
       if (cur.valid) {
@@ -155,8 +156,9 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
     __asm__ __volatile__("" ::: "memory");
   };
 
+  const CsrBufs CB{};  // unused: plain global loads
   CsrPass<FT, DMAX> cur;
-  csr_load_pass<FT, DMAX>(a, s_lp, 0, 64 * w, lane, cur);
+  csr_load_pass<FT, DMAX, false>(a, CB, s_lp, 0, 64 * w, lane, cur);
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
         }
       }
       CsrPass<FT, DMAX> nxt;
-      csr_load_pass<FT, DMAX>(a, s_lp, nlv, nbase, lane, nxt);
+      csr_load_pass<FT, DMAX, false>(a, CB, s_lp, nlv, nbase, lane, nxt);
       bool flipped = false;
       if (cur.valid) {
         FT f = cur.hv;

@@ -105,8 +105,9 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
         if constexpr (DMAX <= 8) {
           // the next pass's neighbour lists in flight while this one runs
           int lv = 0, base = 0;
+          const CsrBufs CB = csr_bufs<FT>(a);
           CsrPass<FT, DMAX> cur;
-          csr_load_pass<FT, DMAX>(a, a.lvl_ptr, 0, 0, lane, cur);
+          csr_load_pass<FT, DMAX, true>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
           while (true) {
             const int b1 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv + 1]);
             int nlv = lv, nbase = base + 64;
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
             }
             const bool last = nlv >= a.nlev;
             CsrPass<FT, DMAX> nxt;
-            if (!last) csr_load_pass<FT, DMAX>(a, a.lvl_ptr, nlv, nbase, lane, nxt);
+            if (!last) csr_load_pass<FT, DMAX, true>(a, CB, a.lvl_ptr, nlv, nbase, lane, nxt);
             bool flipped = false;
             if (cur.valid) {
               FT f = cur.hv;

@@ -20,23 +20,68 @@ struct CsrPass {
   FT val[DMAX];
 };
 
+// The level-order arrays as buffer resources: a pass's loads then share one
+// per-lane byte offset (its position) and take the ELL row d as a scalar
+// offset, so no per-load address arithmetic runs on the vector unit.
+struct CsrBufs {
+  __amdgpu_buffer_rsrc_t order, h, col, val;
+};
+
+template <typename FT>
+__device__ __forceinline__ CsrBufs csr_bufs(const CsrArgs& a) {
+  constexpr int flags = 0x00020000;
+  const int n = a.n, fb = (int)sizeof(FT);
+  CsrBufs b;
+  b.order = __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(a.order), (short)0, n * 4, flags);
+  b.h = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.h), (short)0, n * fb, flags);
+  b.col = __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(a.ell_col), (short)0, a.D * n * 4,
+                                            flags);
+  b.val = __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(a.val), (short)0, a.D * n * fb, flags);
+  return b;
+}
+
+template <typename FT>
+__device__ __forceinline__ FT csr_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(FT) == 8) {
+    return __builtin_bit_cast(FT, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  } else {
+    return __builtin_bit_cast(FT, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  }
+}
+
 // lp: the level pointers (global memory, or a copy in LDS); lv is
-// wave-uniform, and so is the level end read from it.
-template <typename FT, int DMAX>
-__device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const int32_t* lp, int lv,
-                                              int base, int lane, CsrPass<FT, DMAX>& P) {
+// wave-uniform, and so is the level end read from it.  BUF: loads through
+// the buffer resources B (the Trotter kernel: EA3D L = 16, M = 8, R = 1024
+// 4.7 ms against 5.4); else plain global loads (the classical kernels: the
+// workgroup form, with a barrier per level, took 111 ms against 88 on EA3D
+// L = 16 with buffer loads).
+template <typename FT, int DMAX, bool BUF>
+__device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B,
+                                              const int32_t* lp, int lv, int base, int lane,
+                                              CsrPass<FT, DMAX>& P) {
   const int b1 = __builtin_amdgcn_readfirstlane(lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
   const int pos = base + lane;
   P.valid = pos < b1;
   const int pc = P.valid ? pos : b1 - 1;  // levels are never empty
-  const int n = a.n;
-  P.i = a.order[pc];
-  P.hv = reinterpret_cast<const FT*>(a.h)[pc];
-  const FT* ev = reinterpret_cast<const FT*>(a.val);
+  if constexpr (BUF) {
+    const int n = __builtin_amdgcn_readfirstlane(a.n);
+    P.i = csr_ld<int32_t>(B.order, pc * 4, 0);
+    P.hv = csr_ld<FT>(B.h, pc * (int)sizeof(FT), 0);
 #pragma unroll
-  for (int d = 0; d < DMAX; ++d) {
-    P.col[d] = a.ell_col[(int64_t)d * n + pc];
-    P.val[d] = ev[(int64_t)d * n + pc];
+    for (int d = 0; d < DMAX; ++d) {
+      P.col[d] = csr_ld<int32_t>(B.col, pc * 4, d * n * 4);
+      P.val[d] = csr_ld<FT>(B.val, pc * (int)sizeof(FT), d * n * (int)sizeof(FT));
+    }
+  } else {
+    const int n = a.n;
+    P.i = a.order[pc];
+    P.hv = reinterpret_cast<const FT*>(a.h)[pc];
+    const FT* ev = reinterpret_cast<const FT*>(a.val);
+#pragma unroll
+    for (int d = 0; d < DMAX; ++d) {
+      P.col[d] = a.ell_col[(int64_t)d * n + pc];
+      P.val[d] = ev[(int64_t)d * n + pc];
+    }
   }
 }
 
