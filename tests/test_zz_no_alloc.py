@@ -60,6 +60,25 @@ def test_static_energies_of_sparse_contexts_on_two_calls():
             assert got[r] == orc.energy_exact_csr(rp, ci, v, n, init[r])
 
 
+def test_static_energies_of_sparse_trotter_contexts_on_two_calls():
+    """Trotter contexts on the CSR Trotter kernel: one energy per (replica,
+    slice) from the same level-order energy kernel."""
+    for L, R, M in ((8, 40, 6), (4, 3, 3)):  # larger first
+        n, rp, ci, v = H.ea3d_csr(L, 0xA210 + L)
+        init = np.stack([H.random_pm1(M, n, L + r) for r in range(R)])
+        with capi.Context.csr(n, rp, ci, v, R, M, 1.0, capi.PREC_FP64) as ctx:
+            assert ctx.info()["kernel"] == capi.KERNEL_SQA_CSR
+            ctx.set_spins(init)
+            got = static_energies(ctx)
+            want = ctx.energies()
+            again = static_energies(ctx)
+        np.testing.assert_array_equal(got, want)
+        np.testing.assert_array_equal(again, want)
+        for r in range(R):
+            for t in range(M):
+                assert got[r, t] == orc.energy_exact_csr(rp, ci, v, n, init[r, t])
+
+
 def test_static_energies_reject_contexts_beyond_the_scratch():
     E = H.sk_pm1(4096, 1)
     with capi.Context.dense(E, 4096, 1, 0.0, capi.PREC_FP64) as ctx:
