@@ -775,7 +775,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     const size_t spin_words = (size_t)c->M * (size_t)c->NS * 8;
     const size_t fb = c->ftype == FT_F64 ? 8 : 4;
     int kc = 0;
-    if (c->sqa_rows_lds && c->NS <= 8) {
+    if (c->sqa_rows_lds && c->NS <= 8 && c->sqa_ppw < 8) {  // (8 pairs: rows from L2)
       const int64_t per_cu = std::min<int64_t>(2, (c->R + c->num_cus - 1) / c->num_cus);
       kc = sqa_lds_slots(c->ftype, (int)c->NS, (int)c->n, (int)c->M,
                          (size_t)c->lds_budget / (size_t)std::max<int64_t>(1, per_cu));

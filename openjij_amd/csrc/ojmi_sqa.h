@@ -110,39 +110,27 @@ __global__ __launch_bounds__(MAXT) void k_sqa_wave(DenseArgs a) {
               const uint64_t bu = s_down[tu * NS + k], bd = s_down[td * NS + k];
               // s_up + s_dn = 2 - 2(u + d)
               const int nb2 = 2 - 2 * (int)(((bu >> lane) & 1ull) + ((bd >> lane) & 1ull));
-              // A lane leaves `live` when it flips (and so do the lanes below
-              // a flip), so every live lane keeps its own spin, and with it
-              // the Trotter term, for the whole slot: s2 and x = B dEtr are
-              // formed once per slot, dEcl = s2 F per attempt, as before.
-              // (With 8 slice pairs per wave the per-attempt forms keep fewer
-              // values live across the pairs' loops, so they stay there.)
-              const bool dn = (dw >> k) & 1ull;
-              const FT s2 = dn ? (FT)2 : (FT)(-2);
-              const FT xs = Bt * (FT)(dn ? 2 * nb2 : -2 * nb2);
               uint64_t flips = 0;
-              auto candidates = [&]() -> uint64_t {
-                FT dE;
-                if constexpr (PPW < 8) {
-                  dE = fma_ft<FT>(at, s2 * Ft.dyn(k), xs);
-                } else {
-                  const FT fk = Ft.dyn(k);
-                  const bool dl = (sb >> lane) & 1ull;
-                  const FT dEcl = dl ? (FT)2 * fk : (FT)(-2) * fk;
-                  const FT dEtr = (FT)(dl ? 2 * nb2 : -2 * nb2);
-                  dE = fma_ft<FT>(at, dEcl, Bt * dEtr);
-                }
-                return ballot64(dE < th) & live;
-              };
-              // The chain of one slot runs on the scalar unit, which the
-              // waves of a SIMD share: the lowest candidate and the lanes at
-              // or below it come from one subtraction (below = bits 0..i0),
-              // the flip's coefficient is the flipping lane's own s2 (its
-              // spin before the flip), the slot's spin word and flip count
-              // are formed once per slot.
-              // (64 slots, or 8 slice pairs per wave: this form moves the
-              // fields to scratch, so those keep the per-flip updates of sb,
-              // dw and the count.)
+              // (64 slots, or 8 slice pairs per wave: the first form moves
+              // the fields to scratch, so those take the second, with per-flip
+              // updates of sb, dw and the count.)
               if constexpr (NS <= 32 && PPW < 8) {
+                // A lane leaves `live` when it flips (and so do the lanes
+                // below a flip), so every live lane keeps its own spin, and
+                // with it the Trotter term, for the whole slot: s2 and
+                // xs = B dEtr are formed once per slot, dEcl = s2 F per
+                // attempt.  The chain runs on the scalar unit, which the
+                // waves of a SIMD share: the lowest candidate and the lanes
+                // at or below it come from one subtraction (below = bits
+                // 0..i0), the flip's coefficient is the flipping lane's own
+                // s2 (its spin before the flip), and the slot's spin word and
+                // flip count are formed once per slot.
+                const bool dn = (dw >> k) & 1ull;
+                const FT s2 = dn ? (FT)2 : (FT)(-2);
+                const FT xs = Bt * (FT)(dn ? 2 * nb2 : -2 * nb2);
+                auto candidates = [&]() -> uint64_t {
+                  return ballot64(fma_ft<FT>(at, s2 * Ft.dyn(k), xs) < th) & live;
+                };
                 uint64_t cand = candidates();
                 while (cand != 0) {
                   const uint64_t below = cand ^ (cand - 1);
