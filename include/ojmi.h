@@ -192,6 +192,10 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *                    rows from there, 0 reads them from L2
  *   "lds_rows_wpb":  replicas per workgroup of that form (0 = by R: 1 up to
  *                    256 replicas, then up to 4)
+ *   "csr_pack8":     sparse INT32 contexts whose couplings lie in -127..127:
+ *                    1 (default) reads the couplings packed four per word as
+ *                    int8 and forms fields with 4-way integer dot products;
+ *                    0 reads int32 couplings; results are identical
  *   "sqa_rows_lds":  dense Trotter contexts of <= 512 spins: 1 (default)
  *                    keeps the first 1, 2 or 4 slots (64 columns each) of
  *                    every coupling row in LDS, as many as fit beside the

@@ -34,7 +34,7 @@ struct CsrThresh<float> {
   typedef float type;
 };
 
-template <typename FT, int DMAX>
+template <typename FT, int DMAX, bool V8>
 __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   typedef typename CsrThresh<FT>::type TT;
   constexpr bool PREFETCH = DMAX <= 16;
@@ -56,8 +56,8 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
 
   const CsrBufs CB{};  // unused: plain global loads
-  CsrPass<FT, DMAX> cur;
-  if constexpr (PREFETCH) csr_load_pass<FT, DMAX, false>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
+  CsrPass<FT, DMAX, V8> cur;
+  if constexpr (PREFETCH) csr_load_pass<FT, DMAX, false, V8>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
@@ -85,17 +85,15 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
         nlv = 0;
         nbase = 0;
       }
-      CsrPass<FT, DMAX> nxt;
+      CsrPass<FT, DMAX, V8> nxt;
       if constexpr (PREFETCH) {
-        csr_load_pass<FT, DMAX, false>(a, CB, a.lvl_ptr, nlv, nbase, lane, nxt);
+        csr_load_pass<FT, DMAX, false, V8>(a, CB, a.lvl_ptr, nlv, nbase, lane, nxt);
       } else {
-        csr_load_pass<FT, DMAX, false>(a, CB, a.lvl_ptr, lv, base, lane, cur);
+        csr_load_pass<FT, DMAX, false, V8>(a, CB, a.lvl_ptr, lv, base, lane, cur);
       }
       bool flipped = false;
       if (cur.valid) {
-        FT f = cur.hv;
-#pragma unroll
-        for (int d = 0; d < DMAX; ++d) f = (s_s[cur.col[d]] < 0) ? f - cur.val[d] : f + cur.val[d];
+        const FT f = csr_field(cur, s_s);
         const int i = cur.i;
         const int8_t si = s_s[i];
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
@@ -121,7 +119,7 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
 // waited for, so the prefetched neighbour lists of the next pass stay in
 // flight).  Spins of one level are never neighbours, so the order of the
 // passes inside a level does not matter.
-template <typename FT, int DMAX, int W>
+template <typename FT, int DMAX, int W, bool V8>
 __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   typedef typename CsrThresh<FT>::type TT;
   static_assert(DMAX <= 16, "the workgroup form prefetches the next pass");
@@ -157,8 +155,8 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   };
 
   const CsrBufs CB{};  // unused: plain global loads
-  CsrPass<FT, DMAX> cur;
-  csr_load_pass<FT, DMAX, false>(a, CB, s_lp, 0, 64 * w, lane, cur);
+  CsrPass<FT, DMAX, V8> cur;
+  csr_load_pass<FT, DMAX, false, V8>(a, CB, s_lp, 0, 64 * w, lane, cur);
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
@@ -190,13 +188,11 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
           nbase = b1 + 64 * w;
         }
       }
-      CsrPass<FT, DMAX> nxt;
-      csr_load_pass<FT, DMAX, false>(a, CB, s_lp, nlv, nbase, lane, nxt);
+      CsrPass<FT, DMAX, V8> nxt;
+      csr_load_pass<FT, DMAX, false, V8>(a, CB, s_lp, nlv, nbase, lane, nxt);
       bool flipped = false;
       if (cur.valid) {
-        FT f = cur.hv;
-#pragma unroll
-        for (int d = 0; d < DMAX; ++d) f = (s_s[cur.col[d]] < 0) ? f - cur.val[d] : f + cur.val[d];
+        const FT f = csr_field(cur, s_s);
         const int i = cur.i;
         const int8_t si = s_s[i];
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
@@ -220,14 +216,14 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
 // that holds the maximum degree D.
 int csr_padded_width(int D) { return D <= 8 ? 8 : (D <= 16 ? 16 : (D <= 32 ? 32 : 64)); }
 
-template <typename FT>
+template <typename FT, bool V8>
 static hipError_t launch_csr_ft(const CsrArgs& a, dim3 grid, dim3 block, size_t lds,
                                 hipStream_t st) {
   switch (a.D) {
-    case 8: hipLaunchKernelGGL((k_csr_level<FT, 8>), grid, block, lds, st, a); break;
-    case 16: hipLaunchKernelGGL((k_csr_level<FT, 16>), grid, block, lds, st, a); break;
-    case 32: hipLaunchKernelGGL((k_csr_level<FT, 32>), grid, block, lds, st, a); break;
-    case 64: hipLaunchKernelGGL((k_csr_level<FT, 64>), grid, block, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((k_csr_level<FT, 8, V8>), grid, block, lds, st, a); break;
+    case 16: hipLaunchKernelGGL((k_csr_level<FT, 16, V8>), grid, block, lds, st, a); break;
+    case 32: hipLaunchKernelGGL((k_csr_level<FT, 32, V8>), grid, block, lds, st, a); break;
+    case 64: hipLaunchKernelGGL((k_csr_level<FT, 64, V8>), grid, block, lds, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -241,11 +237,11 @@ static size_t csr_wg_lds_bytes(int n, int nlev, size_t tbytes) {
 }
 constexpr size_t CSR_WG_LDS_MAX = 160 * 1024;  // LDS of one gfx950 workgroup
 
-template <typename FT, int DMAX, int W>
+template <typename FT, int DMAX, int W, bool V8>
 static hipError_t launch_csr_wg_d(const CsrArgs& a, hipStream_t st) {
   const size_t lds = csr_wg_lds_bytes(a.n, a.nlev, sizeof(typename CsrThresh<FT>::type));
   if (lds > CSR_WG_LDS_MAX) return hipErrorInvalidValue;
-  auto kern = k_csr_wg<FT, DMAX, W>;
+  auto kern = k_csr_wg<FT, DMAX, W, V8>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
@@ -253,18 +249,18 @@ static hipError_t launch_csr_wg_d(const CsrArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <typename FT, int W>
+template <typename FT, int W, bool V8>
 static hipError_t launch_csr_wg_t(const CsrArgs& a, hipStream_t st) {
-  if (a.D == 8) return launch_csr_wg_d<FT, 8, W>(a, st);
-  if (a.D == 16) return launch_csr_wg_d<FT, 16, W>(a, st);
+  if (a.D == 8) return launch_csr_wg_d<FT, 8, W, V8>(a, st);
+  if (a.D == 16) return launch_csr_wg_d<FT, 16, W, V8>(a, st);
   return hipErrorInvalidValue;
 }
 
-template <typename FT>
+template <typename FT, bool V8 = false>
 static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
-  if (W == 2) return launch_csr_wg_t<FT, 2>(a, st);
-  if (W == 4) return launch_csr_wg_t<FT, 4>(a, st);
-  if (W == 8) return launch_csr_wg_t<FT, 8>(a, st);
+  if (W == 2) return launch_csr_wg_t<FT, 2, V8>(a, st);
+  if (W == 4) return launch_csr_wg_t<FT, 4, V8>(a, st);
+  if (W == 8) return launch_csr_wg_t<FT, 8, V8>(a, st);
   return hipErrorInvalidValue;
 }
 
@@ -290,15 +286,17 @@ hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hip
   if (W > 1) {
     if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
     if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);
+    if (ftype == FT_I32 && a.val8) return launch_csr_wg_ft<int32_t, true>(a, W, st);
     if (ftype == FT_I32) return launch_csr_wg_ft<int32_t>(a, W, st);
     return hipErrorInvalidValue;
   }
   const int ng = (a.n + 3) >> 2;
   const size_t lds = (size_t)wpb * ((size_t)ng * 20);
   dim3 grid((a.R + wpb - 1) / wpb), block(64 * wpb);
-  if (ftype == FT_F64) return launch_csr_ft<double>(a, grid, block, lds, st);
-  if (ftype == FT_F32) return launch_csr_ft<float>(a, grid, block, lds, st);
-  if (ftype == FT_I32) return launch_csr_ft<int32_t>(a, grid, block, lds, st);
+  if (ftype == FT_F64) return launch_csr_ft<double, false>(a, grid, block, lds, st);
+  if (ftype == FT_F32) return launch_csr_ft<float, false>(a, grid, block, lds, st);
+  if (ftype == FT_I32 && a.val8) return launch_csr_ft<int32_t, true>(a, grid, block, lds, st);
+  if (ftype == FT_I32) return launch_csr_ft<int32_t, false>(a, grid, block, lds, st);
   return hipErrorInvalidValue;
 }
 

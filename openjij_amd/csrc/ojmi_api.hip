@@ -95,6 +95,7 @@ struct ojmi_ctx {
   int mfma_accept = 85;  // per mille: the matrix-core sweep at or above this rate
   int mfma_form = 1;     // 1: four waves update the fields while four decide; 0: all do both
   int sqa_rows_lds = 1;  // Trotter kernel: first slots of every coupling row in LDS (0: none)
+  int csr_pack8 = 1;     // INT32 CSR sweeps: int8 values packed four per word (0: int32 values)
   void* dJpk = nullptr;  // packed MFMA operands of the couplings (k_dense_mfma)
   int32_t* dsel = nullptr;  // segment kernel choice and flip snapshot (mode 2)
   int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
@@ -119,6 +120,7 @@ struct ojmi_ctx {
   int32_t *d_order = nullptr, *d_lvl = nullptr, *d_deg = nullptr, *d_ell = nullptr;
   void* d_ellval = nullptr;
   double* d_ellv64 = nullptr;
+  uint32_t* d_ell8 = nullptr;  // INT32 CSR: values in int8, four per word ([Dp/4][n]), or null
   double* d_hpos64 = nullptr;
   int nlev = 0, D = 0;
 
@@ -179,7 +181,7 @@ struct ojmi_ctx {
     if (d_ellv64 == d_ellval) d_ellv64 = nullptr;
     if (d_hpos64 == dh) d_hpos64 = nullptr;
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
-                    d_ellv64, d_hpos64,
+                    d_ellv64, d_hpos64, d_ell8,
                     dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
                     d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
                     dJhi, dJlo, dS8, dSbf, dpart, dstats, d_q_items, d_q_begin, d_q_head,
@@ -703,6 +705,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.p0_f = c->dp0_f + t_begin;
     a.rule = c->rule;
     a.stats = c->dstats;
+    a.val8 = c->csr_pack8 ? c->d_ell8 : nullptr;
     return launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   }
   if (c->family == OJMI_KERNEL_SQA_CSR) {
@@ -1201,6 +1204,29 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
     delete c;
     return fail(OJMI_EDEVICE, "hipMemcpy of CSR values failed");
   }
+  // INT32 sweeps of couplings within int8: the values also packed four per
+  // word (word (q, pos) holds entries 4q .. 4q+3 of position pos, padding 0),
+  // for the dot4 field form of the CSR kernels
+  if (c->ftype == FT_I32 && !sqa_csr) {
+    bool fits = true;
+    for (double v : ellv) fits = fits && v >= -127.0 && v <= 127.0;
+    if (fits) {
+      std::vector<uint32_t> p8((size_t)(Dp / 4 * n), 0u);
+      for (int64_t d = 0; d < Dp; ++d)
+        for (int64_t pos = 0; pos < n; ++pos) {
+          const uint32_t b = (uint32_t)(uint8_t)(int8_t)(int32_t)ellv[d * n + pos];
+          p8[(d / 4) * n + pos] |= b << (8 * (d % 4));
+        }
+      if ((rc = dalloc(&c->d_ell8, p8.size()))) {
+        delete c;
+        return rc;
+      }
+      if (hipMemcpy(c->d_ell8, p8.data(), p8.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        delete c;
+        return fail(OJMI_EDEVICE, "hipMemcpy of packed CSR values failed");
+      }
+    }
+  }
   // energies run over the same ELL arrays with FP64 values (k_csr_energy);
   // FP32 and int32 contexts keep an FP64 copy of the values (Dp x n)
   if (c->ftype == FT_F64) {
@@ -1538,6 +1564,11 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   if (k == "mfma_form") {
     if (value != 0 && value != 1) return fail(OJMI_EINVAL, "mfma_form must be 0 or 1");
     c->mfma_form = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "csr_pack8") {
+    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "csr_pack8 must be 0 or 1");
+    c->csr_pack8 = (int)value;
     return OJMI_OK;
   }
   if (k == "sqa_rows_lds") {

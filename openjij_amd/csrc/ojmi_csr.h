@@ -11,13 +11,18 @@ namespace ojmi {
 // can be in flight while the current one is processed (DMAX <= 16; wider
 // lists are loaded at the start of their own pass).  A zero coupling adds +-0
 // to the field, which changes no decision.
-template <typename FT, int DMAX>
+// V8 (INT32 fields, couplings within int8): the values of a pass are
+// DMAX / 4 words of four packed int8, and the field is formed by dot4
+// instructions on the gathered neighbour spins packed the same way (exact
+// integer sums, so the same field as the per-neighbour adds).
+template <typename FT, int DMAX, bool V8 = false>
 struct CsrPass {
   int i;        // spin index of this lane's position
   bool valid;   // position inside the level
   FT hv;
   int col[DMAX];
-  FT val[DMAX];
+  FT val[V8 ? 1 : DMAX];
+  uint32_t v8[V8 ? DMAX / 4 : 1];
 };
 
 // The level-order arrays as buffer resources: a pass's loads then share one
@@ -55,10 +60,11 @@ __device__ __forceinline__ FT csr_ld(__amdgpu_buffer_rsrc_t r, int voff, int sof
 // 4.7 ms against 5.4); else plain global loads (the classical kernels: the
 // workgroup form, with a barrier per level, took 111 ms against 88 on EA3D
 // L = 16 with buffer loads).
-template <typename FT, int DMAX, bool BUF>
+template <typename FT, int DMAX, bool BUF, bool V8 = false>
 __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B,
                                               const int32_t* lp, int lv, int base, int lane,
-                                              CsrPass<FT, DMAX>& P) {
+                                              CsrPass<FT, DMAX, V8>& P) {
+  static_assert(!(BUF && V8), "packed values are loaded by the global-load form");
   const int b1 = __builtin_amdgcn_readfirstlane(lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
   const int pos = base + lane;
   P.valid = pos < b1;
@@ -76,13 +82,38 @@ __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B
     const int n = a.n;
     P.i = a.order[pc];
     P.hv = reinterpret_cast<const FT*>(a.h)[pc];
-    const FT* ev = reinterpret_cast<const FT*>(a.val);
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d) {
-      P.col[d] = a.ell_col[(int64_t)d * n + pc];
-      P.val[d] = ev[(int64_t)d * n + pc];
+    for (int d = 0; d < DMAX; ++d) P.col[d] = a.ell_col[(int64_t)d * n + pc];
+    if constexpr (V8) {
+#pragma unroll
+      for (int q = 0; q < DMAX / 4; ++q) P.v8[q] = a.val8[(int64_t)q * n + pc];
+    } else {
+      const FT* ev = reinterpret_cast<const FT*>(a.val);
+#pragma unroll
+      for (int d = 0; d < DMAX; ++d) P.val[d] = ev[(int64_t)d * n + pc];
     }
   }
+}
+
+// The field of a pass's spin from the slice's int8 spins s (stored order).
+template <typename FT, int DMAX, bool V8>
+__device__ __forceinline__ FT csr_field(const CsrPass<FT, DMAX, V8>& P, const int8_t* s) {
+  FT f = P.hv;
+  if constexpr (V8) {
+    static_assert(sizeof(FT) == 4 && (FT)0.5 == (FT)0, "packed values need INT32 fields");
+#pragma unroll
+    for (int q = 0; q < DMAX / 4; ++q) {
+      const uint32_t w = (uint32_t)(uint8_t)s[P.col[4 * q]] |
+                         ((uint32_t)(uint8_t)s[P.col[4 * q + 1]] << 8) |
+                         ((uint32_t)(uint8_t)s[P.col[4 * q + 2]] << 16) |
+                         ((uint32_t)(uint8_t)s[P.col[4 * q + 3]] << 24);
+      f = __builtin_amdgcn_sdot4((int)w, (int)P.v8[q], f, false);
+    }
+  } else {
+#pragma unroll
+    for (int d = 0; d < DMAX; ++d) f = (s[P.col[d]] < 0) ? f - P.val[d] : f + P.val[d];
+  }
+  return f;
 }
 
 }  // namespace ojmi

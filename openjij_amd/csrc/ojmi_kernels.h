@@ -71,6 +71,9 @@ struct CsrArgs {
   unsigned long long* stats = nullptr;  // optional [2]: += accepted flips, coupling rows streamed
   const double* p1_d = nullptr;  // Trotter: B per sweep (p0 = s beta / M)
   const float* p1_f = nullptr;
+  // INT32 fields whose couplings fit int8: the values packed four per word,
+  // word (q, pos) at q * n + pos holding entries 4q .. 4q+3 (else nullptr)
+  const uint32_t* val8 = nullptr;
 };
 
 // Workgroup-shared dense sweep (k_shared.hip): J in natural row layout

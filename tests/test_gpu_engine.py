@@ -321,6 +321,62 @@ def test_csr_chain_many_levels(n, waves):
     np.testing.assert_array_equal(got, want)
 
 
+def int_sparse_csr(n, deg, vmax, seed):
+    """Random sparse graph with integer couplings and fields in -vmax..vmax
+    (both ends always present), as the full symmetric (n+1)-row CSR."""
+    rng = np.random.default_rng(seed)
+    pairs = set()
+    while len(pairs) < n * deg // 2:
+        i, j = rng.integers(0, n, 2)
+        if i != j:
+            pairs.add((min(i, j), max(i, j)))
+    rows = [[] for _ in range(n + 1)]
+    for k, (i, j) in enumerate(sorted(pairs)):
+        v = float(vmax if k == 0 else -vmax if k == 1 else rng.integers(-vmax, vmax + 1))
+        if v != 0:
+            rows[i].append((j, v))
+            rows[j].append((i, v))
+    for i in range(n):
+        hv = float(rng.integers(-vmax, vmax + 1))
+        if hv != 0:
+            rows[i].append((n, hv))
+            rows[n].append((i, hv))
+    rows[n].append((n, 1.0))
+    rp, ci, va = [0], [], []
+    for r in rows:
+        for j, v in sorted(r):
+            ci.append(j)
+            va.append(v)
+        rp.append(len(ci))
+    return np.array(rp, np.int64), np.array(ci, np.int64), np.array(va, float)
+
+
+@pytest.mark.parametrize("deg,vmax,waves", [(2, 127, 0), (2, 127, 4), (5, 127, 2),
+                                            (5, 127, 1), (12, 127, 0), (30, 100, 0),
+                                            (5, 128, 4), (12, 1000, 0)])
+def test_csr_int_couplings_packed_and_plain_bitexact(deg, vmax, waves):
+    """INT32 sparse sweeps with couplings within int8 form fields by 4-way
+    int8 dot products (csr_pack8 = 1, the default); wider couplings, and
+    csr_pack8 = 0, add int32 values one neighbour at a time.  Both equal the
+    oracle's sequential sweep at every degree width (8, 16, 32, 64)."""
+    n = 700
+    rp, ci, v = int_sparse_csr(n, deg, vmax, 0x1B8 + deg + vmax)
+    beta, mcs = sched(8, 0.01, 0.5)
+    R = 6
+    init = H.random_pm1(R, n, deg)
+    want = orc.replay_csr("int32", rp, ci, v, n, init, 808, 2, orc.expand_classical(beta, mcs))
+    for pack in (1, 0):
+        with capi.Context.csr(n, rp, ci, v, R, precision=capi.PREC_INT32) as ctx:
+            assert ctx.info()["kernel"] == capi.KERNEL_CSR_LEVEL
+            ctx.set_option("csr_pack8", pack)
+            if waves:
+                ctx.set_option("csr_waves", waves)
+            ctx.set_spins(init)
+            ctx.run(beta, mcs, 808, 2)
+            got = ctx.get_spins()
+        np.testing.assert_array_equal(got, want, err_msg=f"csr_pack8={pack}")
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_csr_real_couplings_bitexact(prec):
     E = H.sk_gauss(90, 4242, field=True)
