@@ -155,8 +155,8 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   };
 
   const CsrBufs CB{};  // unused: plain global loads
-  CsrPass<FT, DMAX, V8> cur;
-  csr_load_pass<FT, DMAX, false, V8>(a, CB, s_lp, 0, 64 * w, lane, cur);
+  CsrPass<FT, DMAX, V8, true> cur;
+  csr_load_pass<FT, DMAX, false, V8, true, true>(a, CB, s_lp, 0, 64 * w, lane, cur);
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
@@ -188,8 +188,8 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
           nbase = b1 + 64 * w;
         }
       }
-      CsrPass<FT, DMAX, V8> nxt;
-      csr_load_pass<FT, DMAX, false, V8>(a, CB, s_lp, nlv, nbase, lane, nxt);
+      CsrPass<FT, DMAX, V8, true> nxt;
+      csr_load_pass<FT, DMAX, false, V8, true, true>(a, CB, s_lp, nlv, nbase, lane, nxt);
       bool flipped = false;
       if (cur.valid) {
         const FT f = csr_field(cur, s_s);
@@ -284,6 +284,7 @@ hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hip
   if (a.R <= 0 || a.T < 0) return hipSuccess;
   const int W = csr_waves_per_replica(a.R, a.D, waves, a.n, a.nlev, ftype);
   if (W > 1) {
+    if (!a.col16) return hipErrorInvalidValue;  // 16-bit columns: built for every n it takes
     if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
     if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);
     if (ftype == FT_I32 && a.val8) return launch_csr_wg_ft<int32_t, true>(a, W, st);

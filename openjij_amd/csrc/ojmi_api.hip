@@ -121,6 +121,7 @@ struct ojmi_ctx {
   void* d_ellval = nullptr;
   double* d_ellv64 = nullptr;
   uint32_t* d_ell8 = nullptr;  // INT32 CSR: values in int8, four per word ([Dp/4][n]), or null
+  uint32_t* d_col16 = nullptr;  // CSR: columns in 16 bits, two per word ([Dp/2][n]), or null
   double* d_hpos64 = nullptr;
   int nlev = 0, D = 0;
 
@@ -181,7 +182,7 @@ struct ojmi_ctx {
     if (d_ellv64 == d_ellval) d_ellv64 = nullptr;
     if (d_hpos64 == dh) d_hpos64 = nullptr;
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
-                    d_ellv64, d_hpos64, d_ell8,
+                    d_ellv64, d_hpos64, d_ell8, d_col16,
                     dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
                     d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
                     dJhi, dJlo, dS8, dSbf, dpart, dstats, d_q_items, d_q_begin, d_q_head,
@@ -706,6 +707,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.rule = c->rule;
     a.stats = c->dstats;
     a.val8 = c->csr_pack8 ? c->d_ell8 : nullptr;
+    a.col16 = c->d_col16;
     return launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   }
   if (c->family == OJMI_KERNEL_SQA_CSR) {
@@ -1203,6 +1205,21 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
       hipMemcpy(dhb, hb.data(), hb.size(), hipMemcpyHostToDevice) != hipSuccess) {
     delete c;
     return fail(OJMI_EDEVICE, "hipMemcpy of CSR values failed");
+  }
+  // the workgroup form reads 16-bit columns (its LDS holds at most 32768 spins)
+  if (!sqa_csr && Dp <= 16 && n < 65536) {
+    std::vector<uint32_t> p16((size_t)(Dp / 2 * n), 0u);
+    for (int64_t d = 0; d < Dp; ++d)
+      for (int64_t pos = 0; pos < n; ++pos)
+        p16[(d / 2) * n + pos] |= (uint32_t)ell[d * n + pos] << (16 * (d % 2));
+    if ((rc = dalloc(&c->d_col16, p16.size()))) {
+      delete c;
+      return rc;
+    }
+    if (hipMemcpy(c->d_col16, p16.data(), p16.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      delete c;
+      return fail(OJMI_EDEVICE, "hipMemcpy of 16-bit CSR columns failed");
+    }
   }
   // INT32 sweeps of couplings within int8: the values also packed four per
   // word (word (q, pos) holds entries 4q .. 4q+3 of position pos, padding 0),

@@ -15,15 +15,25 @@ namespace ojmi {
 // DMAX / 4 words of four packed int8, and the field is formed by dot4
 // instructions on the gathered neighbour spins packed the same way (exact
 // integer sums, so the same field as the per-neighbour adds).
-template <typename FT, int DMAX, bool V8 = false>
+// C16 (spin indices below 65536): the columns are DMAX / 2 words of two
+// 16-bit indices.  Both forms cut the vector loads of a pass, which bound the
+// workgroup form (texture units 60-70% busy with 18 loads per pass).
+template <typename FT, int DMAX, bool V8 = false, bool C16 = false>
 struct CsrPass {
   int i;        // spin index of this lane's position
   bool valid;   // position inside the level
   FT hv;
-  int col[DMAX];
+  int col[C16 ? 1 : DMAX];
+  uint32_t cw[C16 ? DMAX / 2 : 1];
   FT val[V8 ? 1 : DMAX];
   uint32_t v8[V8 ? DMAX / 4 : 1];
 };
+
+template <typename FT, int DMAX, bool V8, bool C16>
+__device__ __forceinline__ int csr_col(const CsrPass<FT, DMAX, V8, C16>& P, int d) {
+  if constexpr (C16) return (int)((P.cw[d >> 1] >> (16 * (d & 1))) & 0xffffu);
+  else return P.col[d];
+}
 
 // The level-order arrays as buffer resources: a pass's loads then share one
 // per-lane byte offset (its position) and take the ELL row d as a scalar
@@ -55,17 +65,24 @@ __device__ __forceinline__ FT csr_ld(__amdgpu_buffer_rsrc_t r, int voff, int sof
 }
 
 // lp: the level pointers (global memory, or a copy in LDS); lv is
-// wave-uniform, and so is the level end read from it.  BUF: loads through
+// wave-uniform, and so is the level end read from it.  EMPTY: base may lie
+// past the level's end (the workgroup form's waves beyond a short level);
+// such a pass issues no loads (config B: 72.4 -> 67.2 ms; the check costs
+// the wave-per-replica kernel, which has no empty passes, 7%).  BUF: loads through
 // the buffer resources B (the Trotter kernel: EA3D L = 16, M = 8, R = 1024
 // 4.7 ms against 5.4); else plain global loads (the classical kernels: the
 // workgroup form, with a barrier per level, took 111 ms against 88 on EA3D
 // L = 16 with buffer loads).
-template <typename FT, int DMAX, bool BUF, bool V8 = false>
+template <typename FT, int DMAX, bool BUF, bool V8 = false, bool EMPTY = false, bool C16 = false>
 __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B,
                                               const int32_t* lp, int lv, int base, int lane,
-                                              CsrPass<FT, DMAX, V8>& P) {
-  static_assert(!(BUF && V8), "packed values are loaded by the global-load form");
+                                              CsrPass<FT, DMAX, V8, C16>& P) {
+  static_assert(!(BUF && (V8 || C16)), "packed arrays are loaded by the global-load form");
   const int b1 = __builtin_amdgcn_readfirstlane(lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
+  if (EMPTY && base >= b1) {  // an empty pass: no loads
+    P.valid = false;
+    return;
+  }
   const int pos = base + lane;
   P.valid = pos < b1;
   const int pc = P.valid ? pos : b1 - 1;  // levels are never empty
@@ -82,8 +99,13 @@ __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B
     const int n = a.n;
     P.i = a.order[pc];
     P.hv = reinterpret_cast<const FT*>(a.h)[pc];
+    if constexpr (C16) {
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d) P.col[d] = a.ell_col[(int64_t)d * n + pc];
+      for (int q = 0; q < DMAX / 2; ++q) P.cw[q] = a.col16[(int64_t)q * n + pc];
+    } else {
+#pragma unroll
+      for (int d = 0; d < DMAX; ++d) P.col[d] = a.ell_col[(int64_t)d * n + pc];
+    }
     if constexpr (V8) {
 #pragma unroll
       for (int q = 0; q < DMAX / 4; ++q) P.v8[q] = a.val8[(int64_t)q * n + pc];
@@ -96,22 +118,22 @@ __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B
 }
 
 // The field of a pass's spin from the slice's int8 spins s (stored order).
-template <typename FT, int DMAX, bool V8>
-__device__ __forceinline__ FT csr_field(const CsrPass<FT, DMAX, V8>& P, const int8_t* s) {
+template <typename FT, int DMAX, bool V8, bool C16>
+__device__ __forceinline__ FT csr_field(const CsrPass<FT, DMAX, V8, C16>& P, const int8_t* s) {
   FT f = P.hv;
   if constexpr (V8) {
     static_assert(sizeof(FT) == 4 && (FT)0.5 == (FT)0, "packed values need INT32 fields");
 #pragma unroll
     for (int q = 0; q < DMAX / 4; ++q) {
-      const uint32_t w = (uint32_t)(uint8_t)s[P.col[4 * q]] |
-                         ((uint32_t)(uint8_t)s[P.col[4 * q + 1]] << 8) |
-                         ((uint32_t)(uint8_t)s[P.col[4 * q + 2]] << 16) |
-                         ((uint32_t)(uint8_t)s[P.col[4 * q + 3]] << 24);
+      const uint32_t w = (uint32_t)(uint8_t)s[csr_col(P, 4 * q)] |
+                         ((uint32_t)(uint8_t)s[csr_col(P, 4 * q + 1)] << 8) |
+                         ((uint32_t)(uint8_t)s[csr_col(P, 4 * q + 2)] << 16) |
+                         ((uint32_t)(uint8_t)s[csr_col(P, 4 * q + 3)] << 24);
       f = __builtin_amdgcn_sdot4((int)w, (int)P.v8[q], f, false);
     }
   } else {
 #pragma unroll
-    for (int d = 0; d < DMAX; ++d) f = (s[P.col[d]] < 0) ? f - P.val[d] : f + P.val[d];
+    for (int d = 0; d < DMAX; ++d) f = (s[csr_col(P, d)] < 0) ? f - P.val[d] : f + P.val[d];
   }
   return f;
 }

@@ -74,6 +74,9 @@ struct CsrArgs {
   // INT32 fields whose couplings fit int8: the values packed four per word,
   // word (q, pos) at q * n + pos holding entries 4q .. 4q+3 (else nullptr)
   const uint32_t* val8 = nullptr;
+  // the columns as 16-bit indices, two per word ((q, pos) at q * n + pos holds
+  // entries 2q and 2q+1): n < 65536, required by the workgroup form
+  const uint32_t* col16 = nullptr;
 };
 
 // Workgroup-shared dense sweep (k_shared.hip): J in natural row layout
