@@ -284,7 +284,7 @@ hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hip
   if (a.R <= 0 || a.T < 0) return hipSuccess;
   const int W = csr_waves_per_replica(a.R, a.D, waves, a.n, a.nlev, ftype);
   if (W > 1) {
-    if (!a.col16) return hipErrorInvalidValue;  // 16-bit columns: built for every n it takes
+    if (!a.rec) return hipErrorInvalidValue;  // position records: built for every n it takes
     if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
     if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);
     if (ftype == FT_I32 && a.val8) return launch_csr_wg_ft<int32_t, true>(a, W, st);

@@ -121,7 +121,7 @@ struct ojmi_ctx {
   void* d_ellval = nullptr;
   double* d_ellv64 = nullptr;
   uint32_t* d_ell8 = nullptr;  // INT32 CSR: values in int8, four per word ([Dp/4][n]), or null
-  uint32_t* d_col16 = nullptr;  // CSR: columns in 16 bits, two per word ([Dp/2][n]), or null
+  uint4 *d_rec = nullptr, *d_rec8 = nullptr;  // CSR workgroup form: position records, or null
   double* d_hpos64 = nullptr;
   int nlev = 0, D = 0;
 
@@ -182,7 +182,7 @@ struct ojmi_ctx {
     if (d_ellv64 == d_ellval) d_ellv64 = nullptr;
     if (d_hpos64 == dh) d_hpos64 = nullptr;
     void* bufs[] = {dJ, dh, dJ64, dh64, d_order, d_lvl, d_deg, d_ell, d_ellval, dspins,
-                    d_ellv64, d_hpos64, d_ell8, d_col16,
+                    d_ellv64, d_hpos64, d_ell8, d_rec, d_rec8,
                     dE, ddrift, dF, dp0_d, dp1_d, dp0_f, dp1_f, dJnat, dprog,
                     d_bn, d_bj_off, d_bh_off, d_bs_off, d_brb, db_p0_d, db_p0_f,
                     dJhi, dJlo, dS8, dSbf, dpart, dstats, d_q_items, d_q_begin, d_q_head,
@@ -707,7 +707,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.rule = c->rule;
     a.stats = c->dstats;
     a.val8 = c->csr_pack8 ? c->d_ell8 : nullptr;
-    a.col16 = c->d_col16;
+    a.rec = a.val8 ? c->d_rec8 : c->d_rec;
     return launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   }
   if (c->family == OJMI_KERNEL_SQA_CSR) {
@@ -1206,28 +1206,70 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
     delete c;
     return fail(OJMI_EDEVICE, "hipMemcpy of CSR values failed");
   }
-  // the workgroup form reads 16-bit columns (its LDS holds at most 32768 spins)
+  // position records of the workgroup form (its LDS holds at most 32768
+  // spins, so 16-bit columns): per position the column pairs, the values,
+  // the spin index and h, as 16-byte chunks [words/4][n] (the layout of
+  // CsrRec in ojmi_csr.h); INT32 couplings within int8 get a second set with
+  // the values as int8 quadruples
+  auto build_rec = [&](bool v8, uint4** dst) -> int {
+    const int64_t vwords = v8 ? Dp / 4 : Dp * (int64_t)esz / 4;
+    const int64_t kvals = Dp / 2, kidx = kvals + vwords, kh = kidx + 1;
+    const int64_t words = (kh + (int64_t)esz / 4 + 3) / 4 * 4;
+    std::vector<uint32_t> w((size_t)(words * n), 0u);
+    auto at = [&](int64_t pos, int64_t k) -> uint32_t& {
+      return w[(size_t)(((k / 4) * n + pos) * 4 + k % 4)];
+    };
+    for (int64_t pos = 0; pos < n; ++pos) {
+      for (int64_t d = 0; d < Dp; ++d) {
+        at(pos, d / 2) |= (uint32_t)ell[d * n + pos] << (16 * (d % 2));
+        const double v = ellv[d * n + pos];
+        if (v8) {
+          at(pos, kvals + d / 4) |= (uint32_t)(uint8_t)(int8_t)(int32_t)v << (8 * (d % 4));
+        } else if (c->ftype == FT_F64) {
+          uint64_t b;
+          std::memcpy(&b, &v, 8);
+          at(pos, kvals + 2 * d) = (uint32_t)b;
+          at(pos, kvals + 2 * d + 1) = (uint32_t)(b >> 32);
+        } else if (c->ftype == FT_F32) {
+          const float f = (float)v;
+          std::memcpy(&at(pos, kvals + d), &f, 4);
+        } else {
+          at(pos, kvals + d) = (uint32_t)(int32_t)v;
+        }
+      }
+      at(pos, kidx) = (uint32_t)order[pos];
+      const double hv = hpos[pos];
+      if (c->ftype == FT_F64) {
+        uint64_t b;
+        std::memcpy(&b, &hv, 8);
+        at(pos, kh) = (uint32_t)b;
+        at(pos, kh + 1) = (uint32_t)(b >> 32);
+      } else if (c->ftype == FT_F32) {
+        const float f = (float)hv;
+        std::memcpy(&at(pos, kh), &f, 4);
+      } else {
+        at(pos, kh) = (uint32_t)(int32_t)hv;
+      }
+    }
+    int r = dalloc(dst, (size_t)(words / 4 * n));
+    if (r) return r;
+    if (hipMemcpy(*dst, w.data(), w.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(OJMI_EDEVICE, "hipMemcpy of CSR position records failed");
+    return OJMI_OK;
+  };
+  bool fits8 = c->ftype == FT_I32;
+  for (double v : ellv) fits8 = fits8 && v >= -127.0 && v <= 127.0;
   if (!sqa_csr && Dp <= 16 && n < 65536) {
-    std::vector<uint32_t> p16((size_t)(Dp / 2 * n), 0u);
-    for (int64_t d = 0; d < Dp; ++d)
-      for (int64_t pos = 0; pos < n; ++pos)
-        p16[(d / 2) * n + pos] |= (uint32_t)ell[d * n + pos] << (16 * (d % 2));
-    if ((rc = dalloc(&c->d_col16, p16.size()))) {
+    if ((rc = build_rec(false, &c->d_rec)) || (fits8 && (rc = build_rec(true, &c->d_rec8)))) {
       delete c;
       return rc;
-    }
-    if (hipMemcpy(c->d_col16, p16.data(), p16.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
-      delete c;
-      return fail(OJMI_EDEVICE, "hipMemcpy of 16-bit CSR columns failed");
     }
   }
   // INT32 sweeps of couplings within int8: the values also packed four per
   // word (word (q, pos) holds entries 4q .. 4q+3 of position pos, padding 0),
   // for the dot4 field form of the CSR kernels
-  if (c->ftype == FT_I32 && !sqa_csr) {
-    bool fits = true;
-    for (double v : ellv) fits = fits && v >= -127.0 && v <= 127.0;
-    if (fits) {
+  if (!sqa_csr) {
+    if (fits8) {
       std::vector<uint32_t> p8((size_t)(Dp / 4 * n), 0u);
       for (int64_t d = 0; d < Dp; ++d)
         for (int64_t pos = 0; pos < n; ++pos) {

@@ -16,8 +16,9 @@ namespace ojmi {
 // instructions on the gathered neighbour spins packed the same way (exact
 // integer sums, so the same field as the per-neighbour adds).
 // C16 (spin indices below 65536): the columns are DMAX / 2 words of two
-// 16-bit indices.  Both forms cut the vector loads of a pass, which bound the
-// workgroup form (texture units 60-70% busy with 18 loads per pass).
+// 16-bit indices, loaded from the position records below.  Both forms cut
+// the vector memory work of a pass, which bounds the workgroup form (texture
+// units 60-70% busy with 18 dword loads per pass).
 template <typename FT, int DMAX, bool V8 = false, bool C16 = false>
 struct CsrPass {
   int i;        // spin index of this lane's position
@@ -34,6 +35,20 @@ __device__ __forceinline__ int csr_col(const CsrPass<FT, DMAX, V8, C16>& P, int 
   if constexpr (C16) return (int)((P.cw[d >> 1] >> (16 * (d & 1))) & 0xffffu);
   else return P.col[d];
 }
+
+// Position records of the workgroup form (spin indices below 65536): the
+// words of position pos are 16-byte chunks k at rec[k * n + pos], so each
+// chunk of a pass is one coalesced 16-byte load per lane.  Words: the
+// 16-bit column pairs [0, DMAX/2), the values (int8 quadruples if V8, else
+// FT), the spin index, the field h (FT); zero padding to a multiple of 4.
+template <typename FT, int DMAX, bool V8>
+struct CsrRec {
+  static constexpr int kCols = 0;
+  static constexpr int kVals = DMAX / 2;
+  static constexpr int kIdx = kVals + (V8 ? DMAX / 4 : DMAX * (int)sizeof(FT) / 4);
+  static constexpr int kH = kIdx + 1;
+  static constexpr int kWords = (kH + (int)sizeof(FT) / 4 + 3) / 4 * 4;
+};
 
 // The level-order arrays as buffer resources: a pass's loads then share one
 // per-lane byte offset (its position) and take the ELL row d as a scalar
@@ -97,22 +112,49 @@ __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B
     }
   } else {
     const int n = a.n;
-    P.i = a.order[pc];
-    P.hv = reinterpret_cast<const FT*>(a.h)[pc];
     if constexpr (C16) {
+      typedef CsrRec<FT, DMAX, V8> L;
+      uint32_t wd[L::kWords];
 #pragma unroll
-      for (int q = 0; q < DMAX / 2; ++q) P.cw[q] = a.col16[(int64_t)q * n + pc];
+      for (int k = 0; k < L::kWords / 4; ++k) {
+        const uint4 v = a.rec[(int64_t)k * n + pc];
+        wd[4 * k] = v.x;
+        wd[4 * k + 1] = v.y;
+        wd[4 * k + 2] = v.z;
+        wd[4 * k + 3] = v.w;
+      }
+#pragma unroll
+      for (int q = 0; q < DMAX / 2; ++q) P.cw[q] = wd[L::kCols + q];
+      P.i = (int)wd[L::kIdx];
+      if constexpr (sizeof(FT) == 8) {
+        P.hv = __builtin_bit_cast(FT, (uint64_t)wd[L::kH] | ((uint64_t)wd[L::kH + 1] << 32));
+#pragma unroll
+        for (int d = 0; d < DMAX; ++d)
+          P.val[d] = __builtin_bit_cast(
+              FT, (uint64_t)wd[L::kVals + 2 * d] | ((uint64_t)wd[L::kVals + 2 * d + 1] << 32));
+      } else {
+        P.hv = __builtin_bit_cast(FT, wd[L::kH]);
+        if constexpr (V8) {
+#pragma unroll
+          for (int q = 0; q < DMAX / 4; ++q) P.v8[q] = wd[L::kVals + q];
+        } else {
+#pragma unroll
+          for (int d = 0; d < DMAX; ++d) P.val[d] = __builtin_bit_cast(FT, wd[L::kVals + d]);
+        }
+      }
     } else {
+      P.i = a.order[pc];
+      P.hv = reinterpret_cast<const FT*>(a.h)[pc];
 #pragma unroll
       for (int d = 0; d < DMAX; ++d) P.col[d] = a.ell_col[(int64_t)d * n + pc];
-    }
-    if constexpr (V8) {
+      if constexpr (V8) {
 #pragma unroll
-      for (int q = 0; q < DMAX / 4; ++q) P.v8[q] = a.val8[(int64_t)q * n + pc];
-    } else {
-      const FT* ev = reinterpret_cast<const FT*>(a.val);
+        for (int q = 0; q < DMAX / 4; ++q) P.v8[q] = a.val8[(int64_t)q * n + pc];
+      } else {
+        const FT* ev = reinterpret_cast<const FT*>(a.val);
 #pragma unroll
-      for (int d = 0; d < DMAX; ++d) P.val[d] = ev[(int64_t)d * n + pc];
+        for (int d = 0; d < DMAX; ++d) P.val[d] = ev[(int64_t)d * n + pc];
+      }
     }
   }
 }

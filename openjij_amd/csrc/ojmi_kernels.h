@@ -74,9 +74,10 @@ struct CsrArgs {
   // INT32 fields whose couplings fit int8: the values packed four per word,
   // word (q, pos) at q * n + pos holding entries 4q .. 4q+3 (else nullptr)
   const uint32_t* val8 = nullptr;
-  // the columns as 16-bit indices, two per word ((q, pos) at q * n + pos holds
-  // entries 2q and 2q+1): n < 65536, required by the workgroup form
-  const uint32_t* col16 = nullptr;
+  // position records of the workgroup form (n < 65536): 16-byte chunk k of
+  // position pos at k * n + pos; words: 16-bit column pairs, the values (int8
+  // quadruples when val8 is set, else the field type), the spin index, h
+  const uint4* rec = nullptr;
 };
 
 // Workgroup-shared dense sweep (k_shared.hip): J in natural row layout
