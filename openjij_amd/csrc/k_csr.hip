@@ -23,6 +23,8 @@
 #include "ojmi_device.h"
 #include "ojmi_csr.h"
 
+#include <type_traits>
+
 namespace ojmi {
 
 template <typename FT>
@@ -32,6 +34,17 @@ struct CsrThresh {
 template <>
 struct CsrThresh<float> {
   typedef float type;
+};
+
+// Thresholds of the workgroup form: INT32 fields hold ceil(theta) as int32
+// (int_threshold), half the LDS of a double and an integer compare.
+template <typename FT>
+struct WgThresh {
+  typedef typename CsrThresh<FT>::type type;
+};
+template <>
+struct WgThresh<int32_t> {
+  typedef int32_t type;
 };
 
 template <typename FT, int DMAX, bool V8>
@@ -121,7 +134,8 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
 // passes inside a level does not matter.
 template <typename FT, int DMAX, int W, bool V8>
 __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
-  typedef typename CsrThresh<FT>::type TT;
+  typedef typename WgThresh<FT>::type TT;
+  typedef typename CsrThresh<FT>::type NB;  // the sweep's scale and exact thresholds
   static_assert(DMAX <= 16, "the workgroup form prefetches the next pass");
   extern __shared__ char smem[];
   const int tid = threadIdx.x;
@@ -160,16 +174,20 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
-    TT nb;
-    if constexpr (sizeof(TT) == 8) nb = sweep_scalar(a.p0_d[t]);
+    NB nb;
+    if constexpr (sizeof(NB) == 8) nb = sweep_scalar(a.p0_d[t]);
     else nb = sweep_scalar(a.p0_f[t]);
+    auto thresh = [&](uint32_t u) -> TT {
+      if constexpr (std::is_same<TT, int32_t>::value) return int_threshold(u, a.rule, nb);
+      else return accept_log<TT>(u, a.rule) * nb;
+    };
     for (int g = tid; g < ng; g += 64 * W) {
       uint32_t w0, w1, w2, w3;
       philox4x32_10((uint32_t)g, tg, rep, 0u, slo, shi, w0, w1, w2, w3);
-      s_t[4 * g + 0] = accept_log<TT>(w0, a.rule) * nb;
-      s_t[4 * g + 1] = accept_log<TT>(w1, a.rule) * nb;
-      s_t[4 * g + 2] = accept_log<TT>(w2, a.rule) * nb;
-      s_t[4 * g + 3] = accept_log<TT>(w3, a.rule) * nb;
+      s_t[4 * g + 0] = thresh(w0);
+      s_t[4 * g + 1] = thresh(w1);
+      s_t[4 * g + 2] = thresh(w2);
+      s_t[4 * g + 3] = thresh(w3);
     }
     bar();  // thresholds of this sweep and spins of the last are visible
     int lv = 0, base = 64 * w;
@@ -231,7 +249,7 @@ static hipError_t launch_csr_ft(const CsrArgs& a, dim3 grid, dim3 block, size_t 
 
 // LDS of the workgroup form: thresholds [4*ng] of `tbytes` each, the int8
 // spins [4*ng] and the level pointers [nlev + 1] (tbytes =
-// sizeof(CsrThresh<FT>::type): 4 for FP32 fields, 8 for FP64 and INT32 fields).
+// sizeof(WgThresh<FT>::type): 4 for FP32 and INT32 fields, 8 for FP64 fields).
 static size_t csr_wg_lds_bytes(int n, int nlev, size_t tbytes) {
   return (size_t)((n + 3) >> 2) * (4 * tbytes + 4) + 4 * ((size_t)nlev + 1);
 }
@@ -239,7 +257,7 @@ constexpr size_t CSR_WG_LDS_MAX = 160 * 1024;  // LDS of one gfx950 workgroup
 
 template <typename FT, int DMAX, int W, bool V8>
 static hipError_t launch_csr_wg_d(const CsrArgs& a, hipStream_t st) {
-  const size_t lds = csr_wg_lds_bytes(a.n, a.nlev, sizeof(typename CsrThresh<FT>::type));
+  const size_t lds = csr_wg_lds_bytes(a.n, a.nlev, sizeof(typename WgThresh<FT>::type));
   if (lds > CSR_WG_LDS_MAX) return hipErrorInvalidValue;
   auto kern = k_csr_wg<FT, DMAX, W, V8>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -271,7 +289,7 @@ static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
 // pointers exceed one workgroup's LDS, take the wave-per-replica kernel.
 int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int ftype) {
   if (D > 16) return 1;
-  const size_t tb = ftype == FT_F32 ? 4 : 8;
+  const size_t tb = ftype == FT_F64 ? 8 : 4;
   if (csr_wg_lds_bytes(n, nlev, tb) > CSR_WG_LDS_MAX) return 1;
   if (requested > 0) return requested;
   // the autotuner's table (openjij_amd/tune.py, ojmi_tuned.h)

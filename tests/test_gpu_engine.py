@@ -305,19 +305,21 @@ def chain_csr(n, seed):
     return np.array(rowptr, np.int64), np.array(col, np.int64), np.array(val, float)
 
 
-@pytest.mark.parametrize("n,waves", [(2000, 4), (13000, 1)])
-def test_csr_chain_many_levels(n, waves):
+@pytest.mark.parametrize("n,waves,prec", [(2000, 4, "int32"), (13000, 4, "int32"),
+                                          (13000, 1, "fp64")])
+def test_csr_chain_many_levels(n, waves, prec):
     """n levels per sweep: the workgroup form keeps the level pointers in LDS
-    when they fit (n = 2000), the wave-per-replica form takes the rest."""
+    when they fit beside its thresholds and spins (9 bytes per spin with INT32
+    thresholds, 13 with FP64 ones), the wave-per-replica form takes the rest."""
     rp, ci, v = chain_csr(n, 0xC4A1 + n)
     beta, mcs = sched(3, 0.3, 2.0)
     init = H.random_pm1(4, n, 21)
-    with capi.Context.csr(n, rp, ci, v, 4, precision=capi.PREC_INT32) as ctx:
+    with capi.Context.csr(n, rp, ci, v, 4, precision=PREC[prec]) as ctx:
         assert ctx.info()["waves_per_replica"] == waves
         ctx.set_spins(init)
         ctx.run(beta, mcs, 55, 0)
         got = ctx.get_spins()
-    want = orc.replay_csr("int32", rp, ci, v, n, init, 55, 0, orc.expand_classical(beta, mcs))
+    want = orc.replay_csr(prec, rp, ci, v, n, init, 55, 0, orc.expand_classical(beta, mcs))
     np.testing.assert_array_equal(got, want)
 
 
@@ -375,6 +377,32 @@ def test_csr_int_couplings_packed_and_plain_bitexact(deg, vmax, waves):
             ctx.run(beta, mcs, 808, 2)
             got = ctx.get_spins()
         np.testing.assert_array_equal(got, want, err_msg=f"csr_pack8={pack}")
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+@pytest.mark.parametrize("waves", [4, 1])
+def test_csr_int_thresholds_at_extreme_betas(rule, waves):
+    """The workgroup form holds INT32 thresholds as ceil(log-factor(U) / -beta)
+    in int32: single-precision logarithms decide unless the bracket straddles
+    an integer, and values beyond int32 clamp.  Betas from 1e-9 (thresholds
+    ~1e10, clamped) to 1e9, with many near-integer thresholds at beta = 1 and
+    0.5, replay the oracle's double comparisons bit for bit, for the
+    Metropolis and heat-bath rules."""
+    n, rp, ci, v = H.ea3d_csr(6, 0x7E5 + rule)
+    v = v * 3.0
+    beta = np.array([1e-9, 1e-6, 0.001, 0.25, 0.5, 1.0, 1.0, 2.0, 1e3, 1e9])
+    mcs = np.full(len(beta), 2, np.int64)
+    R = 40
+    init = H.random_pm1(R, n, 77)
+    with capi.Context.csr(n, rp, ci, v, R, precision=capi.PREC_INT32) as ctx:
+        ctx.set_option("csr_waves", waves)
+        ctx.set_option("update_method", rule)
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 4321, 0)
+        got = ctx.get_spins()
+    want = orc.replay_csr("int32", rp, ci, v, n, init, 4321, 0,
+                          orc.expand_classical(beta, mcs), rule=rule)
+    np.testing.assert_array_equal(got, want)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
