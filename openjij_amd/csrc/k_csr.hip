@@ -169,8 +169,10 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   };
 
   const CsrBufs CB{};  // unused: plain global loads
+  const int nlev = a.nlev;
+  const int end0 = __builtin_amdgcn_readfirstlane(s_lp[1]);  // end of level 0
   CsrPass<FT, DMAX, V8, true> cur;
-  csr_load_pass<FT, DMAX, false, V8, true, true>(a, CB, s_lp, 0, 64 * w, lane, cur);
+  csr_load_pass_at<FT, DMAX, false, V8, true, true>(a, CB, end0, 64 * w, lane, cur);
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
@@ -192,29 +194,37 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
     bar();  // thresholds of this sweep and spins of the last are visible
     int lv = 0, base = 64 * w;
     while (true) {
-      const int b1 = __builtin_amdgcn_readfirstlane(s_lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
-      int nlv = lv, nbase = base + 64 * W;
+      // the ends of this level and the next in one LDS trip
+      const int lq = __builtin_amdgcn_readfirstlane(lv);
+      const int e1v = s_lp[lq + 1], e2v = s_lp[min(lq + 2, nlev)];
+      const int b1 = __builtin_amdgcn_readfirstlane(e1v);
+      const int b2 = __builtin_amdgcn_readfirstlane(e2v);
+      int nlv = lv, nbase = base + 64 * W, nend = b1;
       const bool end_of_level = nbase >= b1;
       bool last = false;
       if (end_of_level) {
         nlv = lv + 1;
-        if (nlv >= a.nlev) {
+        if (nlv >= nlev) {
           last = true;
           nlv = 0;
           nbase = 64 * w;
+          nend = end0;
         } else {
           nbase = b1 + 64 * w;
+          nend = b2;
         }
       }
       CsrPass<FT, DMAX, V8, true> nxt;
-      csr_load_pass<FT, DMAX, false, V8, true, true>(a, CB, s_lp, nlv, nbase, lane, nxt);
+      csr_load_pass_at<FT, DMAX, false, V8, true, true>(a, CB, nend, nbase, lane, nxt);
       bool flipped = false;
       if (cur.valid) {
-        const FT f = csr_field(cur, s_s);
+        // own spin and threshold read with the neighbour gathers: one LDS trip
         const int i = cur.i;
         const int8_t si = s_s[i];
+        const TT th = s_t[i];
+        const FT f = csr_field(cur, s_s);
         const FT dE = (si < 0) ? (FT)2 * f : (FT)(-2) * f;
-        flipped = (TT)dE < s_t[i];
+        flipped = (TT)dE < th;
         if (flipped) s_s[i] = (int8_t)-si;
       }
       nflip += (uint64_t)__builtin_popcountll(ballot64(flipped));

@@ -88,12 +88,12 @@ __device__ __forceinline__ FT csr_ld(__amdgpu_buffer_rsrc_t r, int voff, int sof
 // 4.7 ms against 5.4); else plain global loads (the classical kernels: the
 // workgroup form, with a barrier per level, took 111 ms against 88 on EA3D
 // L = 16 with buffer loads).
+// csr_load_pass_at: the same with the level's end b1 (wave-uniform) given.
 template <typename FT, int DMAX, bool BUF, bool V8 = false, bool EMPTY = false, bool C16 = false>
-__device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B,
-                                              const int32_t* lp, int lv, int base, int lane,
-                                              CsrPass<FT, DMAX, V8, C16>& P) {
+__device__ __forceinline__ void csr_load_pass_at(const CsrArgs& a, const CsrBufs& B, int b1,
+                                                 int base, int lane,
+                                                 CsrPass<FT, DMAX, V8, C16>& P) {
   static_assert(!(BUF && (V8 || C16)), "packed arrays are loaded by the global-load form");
-  const int b1 = __builtin_amdgcn_readfirstlane(lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
   if (EMPTY && base >= b1) {  // an empty pass: no loads
     P.valid = false;
     return;
@@ -157,6 +157,14 @@ __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B
       }
     }
   }
+}
+
+template <typename FT, int DMAX, bool BUF, bool V8 = false, bool EMPTY = false, bool C16 = false>
+__device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B,
+                                              const int32_t* lp, int lv, int base, int lane,
+                                              CsrPass<FT, DMAX, V8, C16>& P) {
+  const int b1 = __builtin_amdgcn_readfirstlane(lp[__builtin_amdgcn_readfirstlane(lv) + 1]);
+  csr_load_pass_at<FT, DMAX, BUF, V8, EMPTY, C16>(a, B, b1, base, lane, P);
 }
 
 // The field of a pass's spin from the slice's int8 spins s (stored order).
