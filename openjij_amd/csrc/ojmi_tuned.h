@@ -14,11 +14,11 @@ struct CsrShape {
   int waves;
 };
 constexpr CsrShape kCsr[] = {
-    {0, 362, 4},
+    {0, 362, 8},
     {0, 724, 4},
-    {0, 1448, 2},
-    {0, 2896, 1},
-    {0, 1099511627776, 1},
+    {0, 1448, 4},
+    {0, 2896, 4},
+    {0, 1099511627776, 4},
     {1, 362, 4},
     {1, 724, 4},
     {1, 1448, 4},
