@@ -136,7 +136,8 @@ template <typename FT, int DMAX, int W, bool V8>
 __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
   typedef typename WgThresh<FT>::type TT;
   typedef typename CsrThresh<FT>::type NB;  // the sweep's scale and exact thresholds
-  static_assert(DMAX <= 16, "the workgroup form prefetches the next pass");
+  static_assert(DMAX <= 16 || (DMAX == 32 && sizeof(FT) == 4),
+                "the workgroup form holds two passes' records in registers");
   extern __shared__ char smem[];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -281,6 +282,9 @@ template <typename FT, int W, bool V8>
 static hipError_t launch_csr_wg_t(const CsrArgs& a, hipStream_t st) {
   if (a.D == 8) return launch_csr_wg_d<FT, 8, W, V8>(a, st);
   if (a.D == 16) return launch_csr_wg_d<FT, 16, W, V8>(a, st);
+  if constexpr (sizeof(FT) == 4) {
+    if (a.D == 32) return launch_csr_wg_d<FT, 32, W, V8>(a, st);
+  }
   return hipErrorInvalidValue;
 }
 
@@ -295,10 +299,11 @@ static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
 // Waves per replica for the workgroup form (1 = the wave-per-replica kernel):
 // several waves split a level's passes, so each SIMD holds more waves and each
 // wave fewer serial passes.  `requested` (1, 2, 4 or 8) overrides the choice;
-// 0 chooses by R.  Graphs of degree > 16, or whose thresholds, spins and level
+// 0 chooses by R.  Graphs of degree > 32 (> 16 with FP64 fields: two passes'
+// records must fit the registers), or whose thresholds, spins and level
 // pointers exceed one workgroup's LDS, take the wave-per-replica kernel.
 int csr_waves_per_replica(int64_t R, int D, int requested, int n, int nlev, int ftype) {
-  if (D > 16) return 1;
+  if (D > 32 || (D > 16 && ftype == FT_F64)) return 1;
   const size_t tb = ftype == FT_F64 ? 8 : 4;
   if (csr_wg_lds_bytes(n, nlev, tb) > CSR_WG_LDS_MAX) return 1;
   if (requested > 0) return requested;

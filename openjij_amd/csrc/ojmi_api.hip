@@ -1259,7 +1259,7 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   };
   bool fits8 = c->ftype == FT_I32;
   for (double v : ellv) fits8 = fits8 && v >= -127.0 && v <= 127.0;
-  if (!sqa_csr && Dp <= 16 && n < 65536) {
+  if (!sqa_csr && Dp <= 32 && n < 65536) {
     if ((rc = build_rec(false, &c->d_rec)) || (fits8 && (rc = build_rec(true, &c->d_rec8)))) {
       delete c;
       return rc;

@@ -8,8 +8,9 @@ namespace ojmi {
 
 // Neighbour lists are padded to DMAX entries (zero couplings on column 0), so
 // every pass loads the same number of values and the lists of the next pass
-// can be in flight while the current one is processed (DMAX <= 16; wider
-// lists are loaded at the start of their own pass).  A zero coupling adds +-0
+// can be in flight while the current one is processed (DMAX <= 16 in the
+// wave-per-replica kernel, 32 in the workgroup form; wider lists are loaded
+// at the start of their own pass).  A zero coupling adds +-0
 // to the field, which changes no decision.
 // V8 (INT32 fields, couplings within int8): the values of a pass are
 // DMAX / 4 words of four packed int8, and the field is formed by dot4

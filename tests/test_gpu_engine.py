@@ -354,8 +354,9 @@ def int_sparse_csr(n, deg, vmax, seed):
 
 
 @pytest.mark.parametrize("deg,vmax,waves", [(2, 127, 0), (2, 127, 4), (5, 127, 2),
-                                            (5, 127, 1), (12, 127, 0), (30, 100, 0),
-                                            (5, 128, 4), (12, 1000, 0)])
+                                            (5, 127, 1), (12, 127, 0), (12, 127, 1),
+                                            (12, 127, 8), (30, 100, 0), (5, 128, 4),
+                                            (12, 1000, 0)])
 def test_csr_int_couplings_packed_and_plain_bitexact(deg, vmax, waves):
     """INT32 sparse sweeps with couplings within int8 form fields by 4-way
     int8 dot products (csr_pack8 = 1, the default); wider couplings, and
@@ -377,6 +378,29 @@ def test_csr_int_couplings_packed_and_plain_bitexact(deg, vmax, waves):
             ctx.run(beta, mcs, 808, 2)
             got = ctx.get_spins()
         np.testing.assert_array_equal(got, want, err_msg=f"csr_pack8={pack}")
+
+
+@pytest.mark.parametrize("prec,waves", [("fp32", 4), ("fp32", 2), ("fp32", 1), ("fp64", 0)])
+def test_csr_degree_32_real_couplings(prec, waves):
+    """Neighbour lists padded to 32: FP32 fields run on the workgroup form
+    (two passes' records in registers), FP64 fields on the wave-per-replica
+    kernel; Gaussian couplings, bit for bit against the oracle."""
+    n = 900
+    rp, ci, v = gauss_sparse_csr(n, 12, 0xD32)
+    assert 16 < np.diff(rp)[:n].max() - 1 <= 32
+    beta, mcs = sched(10, 0.1, 3.0)
+    R = 9
+    init = H.random_pm1(R, n, 32)
+    with capi.Context.csr(n, rp, ci, v, R, precision=PREC[prec]) as ctx:
+        assert ctx.info()["kernel"] == capi.KERNEL_CSR_LEVEL
+        if waves:
+            ctx.set_option("csr_waves", waves)
+        assert ctx.info()["waves_per_replica"] == (waves if prec == "fp32" else 1)
+        ctx.set_spins(init)
+        ctx.run(beta, mcs, 3232, 1)
+        got = ctx.get_spins()
+    want = orc.replay_csr(prec, rp, ci, v, n, init, 3232, 1, orc.expand_classical(beta, mcs))
+    np.testing.assert_array_equal(got, want)
 
 
 @pytest.mark.parametrize("rule", [0, 1])
