@@ -192,6 +192,10 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *                    rows from there, 0 reads them from L2
  *   "lds_rows_wpb":  replicas per workgroup of that form (0 = by R: 1 up to
  *                    256 replicas, then up to 4)
+ *   "sqa_csr_records": sparse Trotter contexts of < 65536 spins and degree
+ *                    <= 8: 1 (default) loads each 64-spin pass's neighbour
+ *                    lists as per-position 16-byte records; 0 loads one word
+ *                    per column and value; results are identical
  *   "csr_pack8":     sparse INT32 contexts whose couplings lie in -127..127:
  *                    1 (default) reads the couplings packed four per word as
  *                    int8 and forms fields with 4-way integer dot products;

@@ -31,7 +31,10 @@ __host__ __device__ inline size_t sqa_csr_spin_bytes(int n, int M) {
 // STAGE: each wave writes the Philox words of its slice's attempts to its own
 // LDS region before the slice's passes (one Philox evaluation per 4 spins
 // instead of one per spin); else every attempt evaluates its own.
-template <typename FT, int DMAX, bool STAGE>
+// REC (neighbour lists of <= 8, n < 65536): a pass loads the per-position
+// records (CsrRec in ojmi_csr.h, 16-byte chunks) instead of one word per
+// column, value, index and field.
+template <typename FT, int DMAX, bool STAGE, bool REC>
 __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
   extern __shared__ int8_t s_spin[];  // [M][n], then (STAGE) words [W][4 ng]
   const int lane = threadIdx.x & 63;
@@ -84,30 +87,35 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
         }
         // one attempt: field from the neighbour list (stored order), the
         // Trotter term from the adjacent slices, the Philox word's threshold
-        auto attempt = [&](int i, FT f) -> bool {
-          const int si = s[i];
-          const FT dEcl = (FT)(-2 * si) * f;
-          const FT dEtr = (FT)(-2 * si * ((int)su[i] + (int)sd[i]));
-          const FT x = Bt * dEtr;
-          const FT dE = fma_ft<FT>(at, dEcl, x);
-          uint32_t u;
+        // (with STAGE, the word and the spin states are read before the
+        // field's gathers, so one LDS round trip serves them all)
+        auto word = [&](int i) -> uint32_t {
           if constexpr (STAGE) {
-            u = s_w[i];
+            return s_w[i];
           } else {
             uint32_t u0, u1, u2, u3;
             philox4x32_10((uint32_t)(i >> 2), tg, rep, (uint32_t)t, slo, shi, u0, u1, u2, u3);
-            u = (i & 2) ? ((i & 1) ? u3 : u2) : ((i & 1) ? u1 : u0);
+            return (i & 2) ? ((i & 1) ? u3 : u2) : ((i & 1) ? u1 : u0);
           }
+        };
+        auto decide = [&](int i, FT f, int si, int sud, uint32_t u) -> bool {
+          const FT dEcl = (FT)(-2 * si) * f;
+          const FT dEtr = (FT)(-2 * si * sud);
+          const FT x = Bt * dEtr;
+          const FT dE = fma_ft<FT>(at, dEcl, x);
           const bool fl = dE < -log_uniform<FT>(u);
           if (fl) s[i] = (int8_t)-si;
           return fl;
         };
+        auto attempt = [&](int i, FT f) -> bool {
+          return decide(i, f, s[i], (int)su[i] + (int)sd[i], word(i));
+        };
         if constexpr (DMAX <= 8) {
           // the next pass's neighbour lists in flight while this one runs
           int lv = 0, base = 0;
-          const CsrBufs CB = csr_bufs<FT>(a);
-          CsrPass<FT, DMAX> cur;
-          csr_load_pass<FT, DMAX, true>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
+          const CsrBufs CB = REC ? CsrBufs{} : csr_bufs<FT>(a);
+          CsrPass<FT, DMAX, false, REC> cur;
+          csr_load_pass<FT, DMAX, !REC, false, false, REC>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
           while (true) {
             const int b1 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv + 1]);
             int nlv = lv, nbase = base + 64;
@@ -116,14 +124,16 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
               nbase = b1;
             }
             const bool last = nlv >= a.nlev;
-            CsrPass<FT, DMAX> nxt;
-            if (!last) csr_load_pass<FT, DMAX, true>(a, CB, a.lvl_ptr, nlv, nbase, lane, nxt);
+            CsrPass<FT, DMAX, false, REC> nxt;
+            if (!last)
+              csr_load_pass<FT, DMAX, !REC, false, false, REC>(a, CB, a.lvl_ptr, nlv, nbase, lane,
+                                                               nxt);
             bool flipped = false;
             if (cur.valid) {
-              FT f = cur.hv;
-#pragma unroll
-              for (int d = 0; d < DMAX; ++d) f = (s[cur.col[d]] < 0) ? f - cur.val[d] : f + cur.val[d];
-              flipped = attempt(cur.i, f);
+              const int i = cur.i;
+              const int si = s[i], sud = (int)su[i] + (int)sd[i];
+              const uint32_t u = word(i);
+              flipped = decide(i, csr_field(cur, s), si, sud, u);
             }
             nflip += (uint64_t)__builtin_popcountll(ballot64(flipped));
             if (last) break;
@@ -177,7 +187,10 @@ template <typename FT, int DMAX>
 static hipError_t launch_sqa_csr_d(const CsrArgs& a, bool stage, dim3 grid, dim3 block,
                                    hipStream_t st) {
   const size_t lds = stage ? sqa_csr_staged_lds_bytes(a.n, a.M) : sqa_csr_lds_bytes(a.n, a.M);
-  auto kern = stage ? k_sqa_csr<FT, DMAX, true> : k_sqa_csr<FT, DMAX, false>;
+  auto kern = stage ? k_sqa_csr<FT, DMAX, true, false> : k_sqa_csr<FT, DMAX, false, false>;
+  if constexpr (DMAX <= 8) {
+    if (a.rec) kern = stage ? k_sqa_csr<FT, DMAX, true, true> : k_sqa_csr<FT, DMAX, false, true>;
+  }
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;

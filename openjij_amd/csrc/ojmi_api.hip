@@ -96,6 +96,7 @@ struct ojmi_ctx {
   int mfma_form = 1;     // 1: four waves update the fields while four decide; 0: all do both
   int sqa_rows_lds = 1;  // Trotter kernel: first slots of every coupling row in LDS (0: none)
   int csr_pack8 = 1;     // INT32 CSR sweeps: int8 values packed four per word (0: int32 values)
+  int sqa_csr_records = 1;  // CSR Trotter kernel: per-position records where built (0: word loads)
   void* dJpk = nullptr;  // packed MFMA operands of the couplings (k_dense_mfma)
   int32_t* dsel = nullptr;  // segment kernel choice and flip snapshot (mode 2)
   int lds_rows = 1;      // wave family: coupling rows staged in LDS when they fit 64 KiB
@@ -734,6 +735,7 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.p1_f = c->dp1_f + t_begin;
     a.rule = RULE_METROPOLIS;
     a.stats = c->dstats;
+    a.rec = c->sqa_csr_records ? c->d_rec : nullptr;
     // The slices' Philox words are staged in LDS (one Philox per 4 spins
     // instead of one per spin) where they fit beside the spins and leave as
     // many workgroups resident per CU as the replicas can use (at most 28
@@ -1259,8 +1261,10 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   };
   bool fits8 = c->ftype == FT_I32;
   for (double v : ellv) fits8 = fits8 && v >= -127.0 && v <= 127.0;
-  if (!sqa_csr && Dp <= 32 && n < 65536) {
-    if ((rc = build_rec(false, &c->d_rec)) || (fits8 && (rc = build_rec(true, &c->d_rec8)))) {
+  // (Trotter systems: the records of neighbour lists of <= 8, k_sqa_csr's prefetching form)
+  if ((sqa_csr ? Dp <= 8 : Dp <= 32) && n < 65536) {
+    if ((rc = build_rec(false, &c->d_rec)) ||
+        (!sqa_csr && fits8 && (rc = build_rec(true, &c->d_rec8)))) {
       delete c;
       return rc;
     }
@@ -1623,6 +1627,11 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
   if (k == "mfma_form") {
     if (value != 0 && value != 1) return fail(OJMI_EINVAL, "mfma_form must be 0 or 1");
     c->mfma_form = (int)value;
+    return OJMI_OK;
+  }
+  if (k == "sqa_csr_records") {
+    if (value != 0 && value != 1) return fail(OJMI_EINVAL, "sqa_csr_records must be 0 or 1");
+    c->sqa_csr_records = (int)value;
     return OJMI_OK;
   }
   if (k == "csr_pack8") {

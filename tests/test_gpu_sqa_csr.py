@@ -94,6 +94,24 @@ def test_sqa_csr_bitexact(kind, size, M, prec, R, L):
         assert list(sol[r]) == list(got[r, int(np.argmin(en.reshape(R, M)[r]))])
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_sqa_csr_records_and_word_loads_agree(prec):
+    """Neighbour lists of <= 8 load as per-position 16-byte records (the
+    default) or one word per column and value (sqa_csr_records = 0): the
+    same spins, equal to the oracle's replay."""
+    n, rp, ci, va = graph("ea3d", (6,), 0x5C0D)
+    M, R, L, gamma = 6, 5, 8, 1.5
+    beta, s, mcs, a, B = trotter_schedule(L, gamma, M)
+    init = np.stack([H.random_pm1(M, n, 29 + r) for r in range(R)])
+    want = orc.replay_sqa_csr(prec, rp, ci, va, n, init, 0x6161, 0, a, B)
+    for rec in (1, 0):
+        with capi.Context.csr(n, rp, ci, va, R, M, gamma, PREC[prec]) as ctx:
+            ctx.set_option("sqa_csr_records", rec)
+            ctx.set_spins(init)
+            ctx.run(beta, mcs, 0x6161, 0, s=s)
+            np.testing.assert_array_equal(ctx.get_spins(), want, err_msg=f"records={rec}")
+
+
 def test_sqa_csr_split_run_equals_one_run():
     n = 500
     rp, ci, va = gauss_sparse_csr(n, 6, 0x5D01)
