@@ -24,7 +24,10 @@ a stated tolerance), the roofline of the dominant kernel from flips and rows
 counted on the GPU, and the CPU baseline (the oracle's restatement of the
 reference algorithm, timed on this host).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config A..E]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--config A..E|T]
+  (T, not part of the default run: SQA on the 3D EA L=16 CSR graph, 8 slices,
+  1024 reads, on the sparse Trotter kernel; CPU baseline: the oracle's FP64
+  replay of the same path)
   python bench.py --calibrate-copy     # plain 256 MiB device copy (HBM counter scale)
 
 Prints one JSON line on stdout; progress goes to stderr.
@@ -107,6 +110,17 @@ def workload(cfg, sweeps=None, replicas=None):
                     s=(4.99 / 5.0) * k / (L - 1), mcs=np.ones(L, np.int64), fast="fp32",
                     plain="fp64", refresh=0, tol_rel=5e-3, integer=False, deg=n + 2,
                     cpu_reads=(8, 64))
+    if cfg == "T":
+        Lside, R, L, M = 16, replicas or 1024, sweeps or 500, 8
+        n, rp, ci, va = H.ea3d_csr(Lside, 0xB002)
+        k = np.arange(L, dtype=np.float64)
+        return dict(cfg=cfg, kind="sqa_csr", name=f"SQA 3D EA L={Lside} +-1 (CSR, N={n}), M={M}, "
+                    f"gamma=1, beta=5, {R} reads, {L} sweeps, s 0->0.998 linear", n=n, R=R, M=M,
+                    gamma=1.0, csr=(rp, ci, va), beta=np.full(L, 5.0),
+                    s=(4.99 / 5.0) * k / (L - 1), mcs=np.ones(L, np.int64), fast="fp32",
+                    plain="fp64", refresh=0, tol_rel=1.5e-2, integer=False, deg=6 + 2,
+                    cpu_reads=(4, 64))  # tol: the reads of 500 sweeps end 1.6% above the
+                                        # 4x-longer anneal's best on average
     if cfg == "D":
         K, R, L = 4096, replicas or 16, sweeps or 1000
         ns = H.qubo_batch_sizes(K, 0xD004)
@@ -140,9 +154,9 @@ def make_ctx(W, prec):
 
     if W["kind"] == "batch":
         return capi.BatchContext.create(W["Es"], W["R"], prec_code(prec))
-    if W["kind"] == "csr":
+    if W["kind"] in ("csr", "sqa_csr"):
         rp, ci, va = W["csr"]
-        return capi.Context.csr(W["n"], rp, ci, va, W["R"], 1, 0.0, prec_code(prec))
+        return capi.Context.csr(W["n"], rp, ci, va, W["R"], W["M"], W["gamma"], prec_code(prec))
     return capi.Context.dense(W["E"], W["R"], W["M"], W["gamma"], prec_code(prec))
 
 
@@ -240,6 +254,17 @@ def _cpu_run(W, reads, threads):
         rp, ci, va = W["csr"]
         orc.ref_sa_csr_batch(rp, ci, va, W["n"], init, seeds, W["beta"], W["mcs"],
                              nthreads=threads)
+    elif W["kind"] == "sqa_csr":
+        # the oracle's FP64 replay of the path; its OpenMP loop over reads
+        # takes the thread count of the calling thread (libgomp's setting)
+        import ctypes
+
+        rp, ci, va = W["csr"]
+        a, B = orc.expand_transverse(W["beta"], W["s"], W["mcs"], W["gamma"], W["M"])
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(threads)
+        t0 = time.perf_counter()
+        orc.replay_sqa_csr("fp64", rp, ci, va, W["n"], init.reshape(reads, W["M"], W["n"]), 5, 0,
+                           a, B)
     else:  # one SQA read per call; ctypes drops the GIL, so reads run in parallel
         Es = H.symmetric_ext(W["E"])
         init = init.reshape(reads, W["M"], W["n"])
@@ -327,7 +352,7 @@ def profiled(kernel_name, workload_name):
 
 KERNEL_NAMES = {1: "k_dense_wave", 2: "k_sqa_wave", 3: "k_csr_level", 4: "k_dense_shared",
                 5: "k_dense_colown", 6: "k_batch_wave", 7: "k_dense_lds", 8: "k_batch_persist",
-                9: "k_batch_wave", 10: "k_dense_mfma"}
+                9: "k_batch_wave", 10: "k_dense_mfma", 11: "k_sqa_csr"}
 
 
 def kernel_name(info):
@@ -360,10 +385,10 @@ def roofline(W, info, sweep_ms, launches, default_run):
     T = int(W["mcs"].sum())
     flips, rows = info["last_flips"], info["last_rows"]
     spins_io = 2 * W["R"] * W["M"] * W["n"]
-    if W["kind"] == "csr":
+    if W["kind"] in ("csr", "sqa_csr"):
         ell = W["n"] * 16 * (4 + bJ)  # level-order ELL arrays, 16 slots wide
         alg = ell + spins_io
-        flop = 2.0 * W["R"] * T * W["n"] * W["deg"]
+        flop = 2.0 * W["R"] * W["M"] * T * W["n"] * W["deg"]
         row_bytes = 0.0
     elif W["kind"] == "batch":
         nsq = float((W["ns"].astype(np.float64) ** 2).sum())
@@ -515,7 +540,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default=None,
-                    help="one config only (A..E); default: E as the headline plus A-D")
+                    help="one config only (A..E, or T: sparse Trotter); default: E as the "
+                         "headline plus A-D")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tts", action="store_true",
