@@ -318,10 +318,15 @@ hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hip
   const int W = csr_waves_per_replica(a.R, a.D, waves, a.n, a.nlev, ftype);
   if (W > 1) {
     if (!a.rec) return hipErrorInvalidValue;  // position records: built for every n it takes
-    if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
-    if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);
-    if (ftype == FT_I32 && a.val8) return launch_csr_wg_ft<int32_t, true>(a, W, st);
-    if (ftype == FT_I32) return launch_csr_wg_ft<int32_t>(a, W, st);
+    if (a.pack8) {
+      if (ftype == FT_F64) return launch_csr_wg_ft<double, true>(a, W, st);
+      if (ftype == FT_F32) return launch_csr_wg_ft<float, true>(a, W, st);
+      if (ftype == FT_I32) return launch_csr_wg_ft<int32_t, true>(a, W, st);
+    } else {
+      if (ftype == FT_F64) return launch_csr_wg_ft<double>(a, W, st);
+      if (ftype == FT_F32) return launch_csr_wg_ft<float>(a, W, st);
+      if (ftype == FT_I32) return launch_csr_wg_ft<int32_t>(a, W, st);
+    }
     return hipErrorInvalidValue;
   }
   const int ng = (a.n + 3) >> 2;

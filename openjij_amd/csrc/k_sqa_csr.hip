@@ -34,7 +34,8 @@ __host__ __device__ inline size_t sqa_csr_spin_bytes(int n, int M) {
 // REC (neighbour lists of <= 8, n < 65536): a pass loads the per-position
 // records (CsrRec in ojmi_csr.h, 16-byte chunks) instead of one word per
 // column, value, index and field.
-template <typename FT, int DMAX, bool STAGE, bool REC>
+// V8 (with REC): the records hold int8 coupling quadruples and integer h.
+template <typename FT, int DMAX, bool STAGE, bool REC, bool V8>
 __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
   extern __shared__ int8_t s_spin[];  // [M][n], then (STAGE) words [W][4 ng]
   const int lane = threadIdx.x & 63;
@@ -114,8 +115,8 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
           // the next pass's neighbour lists in flight while this one runs
           int lv = 0, base = 0;
           const CsrBufs CB = REC ? CsrBufs{} : csr_bufs<FT>(a);
-          CsrPass<FT, DMAX, false, REC> cur;
-          csr_load_pass<FT, DMAX, !REC, false, false, REC>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
+          CsrPass<FT, DMAX, V8, REC> cur;
+          csr_load_pass<FT, DMAX, !REC, V8, false, REC>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
           while (true) {
             const int b1 = __builtin_amdgcn_readfirstlane(a.lvl_ptr[lv + 1]);
             int nlv = lv, nbase = base + 64;
@@ -124,10 +125,10 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
               nbase = b1;
             }
             const bool last = nlv >= a.nlev;
-            CsrPass<FT, DMAX, false, REC> nxt;
+            CsrPass<FT, DMAX, V8, REC> nxt;
             if (!last)
-              csr_load_pass<FT, DMAX, !REC, false, false, REC>(a, CB, a.lvl_ptr, nlv, nbase, lane,
-                                                               nxt);
+              csr_load_pass<FT, DMAX, !REC, V8, false, REC>(a, CB, a.lvl_ptr, nlv, nbase, lane,
+                                                            nxt);
             bool flipped = false;
             if (cur.valid) {
               const int i = cur.i;
@@ -187,9 +188,14 @@ template <typename FT, int DMAX>
 static hipError_t launch_sqa_csr_d(const CsrArgs& a, bool stage, dim3 grid, dim3 block,
                                    hipStream_t st) {
   const size_t lds = stage ? sqa_csr_staged_lds_bytes(a.n, a.M) : sqa_csr_lds_bytes(a.n, a.M);
-  auto kern = stage ? k_sqa_csr<FT, DMAX, true, false> : k_sqa_csr<FT, DMAX, false, false>;
+  auto kern = stage ? k_sqa_csr<FT, DMAX, true, false, false>
+                    : k_sqa_csr<FT, DMAX, false, false, false>;
   if constexpr (DMAX <= 8) {
-    if (a.rec) kern = stage ? k_sqa_csr<FT, DMAX, true, true> : k_sqa_csr<FT, DMAX, false, true>;
+    if (a.rec && a.pack8)
+      kern = stage ? k_sqa_csr<FT, DMAX, true, true, true> : k_sqa_csr<FT, DMAX, false, true, true>;
+    else if (a.rec)
+      kern = stage ? k_sqa_csr<FT, DMAX, true, true, false>
+                   : k_sqa_csr<FT, DMAX, false, true, false>;
   }
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

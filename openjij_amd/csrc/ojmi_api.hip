@@ -708,7 +708,8 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.rule = c->rule;
     a.stats = c->dstats;
     a.val8 = c->csr_pack8 ? c->d_ell8 : nullptr;
-    a.rec = a.val8 ? c->d_rec8 : c->d_rec;
+    a.pack8 = (c->csr_pack8 && c->d_rec8) ? 1 : 0;
+    a.rec = a.pack8 ? c->d_rec8 : c->d_rec;
     return launch_csr_level(c->ftype, a, c->csr_wpb, c->csr_waves, c->st);
   }
   if (c->family == OJMI_KERNEL_SQA_CSR) {
@@ -735,7 +736,8 @@ hipError_t launch_sweeps(ojmi_ctx* c, int64_t t_begin, int64_t T, uint64_t seed,
     a.p1_f = c->dp1_f + t_begin;
     a.rule = RULE_METROPOLIS;
     a.stats = c->dstats;
-    a.rec = c->sqa_csr_records ? c->d_rec : nullptr;
+    a.pack8 = (c->sqa_csr_records && c->csr_pack8 && c->d_rec8) ? 1 : 0;
+    a.rec = !c->sqa_csr_records ? nullptr : (a.pack8 ? c->d_rec8 : c->d_rec);
     // The slices' Philox words are staged in LDS (one Philox per 4 spins
     // instead of one per spin) where they fit beside the spins and leave as
     // many workgroups resident per CU as the replicas can use (at most 28
@@ -1216,7 +1218,7 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   auto build_rec = [&](bool v8, uint4** dst) -> int {
     const int64_t vwords = v8 ? Dp / 4 : Dp * (int64_t)esz / 4;
     const int64_t kvals = Dp / 2, kidx = kvals + vwords, kh = kidx + 1;
-    const int64_t words = (kh + (int64_t)esz / 4 + 3) / 4 * 4;
+    const int64_t words = (kh + (v8 ? 1 : (int64_t)esz / 4) + 3) / 4 * 4;
     std::vector<uint32_t> w((size_t)(words * n), 0u);
     auto at = [&](int64_t pos, int64_t k) -> uint32_t& {
       return w[(size_t)(((k / 4) * n + pos) * 4 + k % 4)];
@@ -1241,16 +1243,16 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
       }
       at(pos, kidx) = (uint32_t)order[pos];
       const double hv = hpos[pos];
-      if (c->ftype == FT_F64) {
+      if (v8 || c->ftype == FT_I32) {
+        at(pos, kh) = (uint32_t)(int32_t)hv;
+      } else if (c->ftype == FT_F64) {
         uint64_t b;
         std::memcpy(&b, &hv, 8);
         at(pos, kh) = (uint32_t)b;
         at(pos, kh + 1) = (uint32_t)(b >> 32);
-      } else if (c->ftype == FT_F32) {
+      } else {
         const float f = (float)hv;
         std::memcpy(&at(pos, kh), &f, 4);
-      } else {
-        at(pos, kh) = (uint32_t)(int32_t)hv;
       }
     }
     int r = dalloc(dst, (size_t)(words / 4 * n));
@@ -1259,12 +1261,17 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
       return fail(OJMI_EDEVICE, "hipMemcpy of CSR position records failed");
     return OJMI_OK;
   };
-  bool fits8 = c->ftype == FT_I32;
-  for (double v : ellv) fits8 = fits8 && v >= -127.0 && v <= 127.0;
+  // Integer couplings within int8 (fitsv) and, for real field types, integer
+  // fields whose sums with them stay exact in the field type (|h| <= 2^20:
+  // below 2^24 with 32 neighbours of |J| <= 127) take the int8 forms
+  bool fitsv = true;
+  for (double v : ellv) fitsv = fitsv && v >= -127.0 && v <= 127.0 && v == std::floor(v);
+  bool fits8 = fitsv;
+  if (c->ftype != FT_I32)
+    for (double hv : hpos) fits8 = fits8 && std::fabs(hv) <= 1048576.0 && hv == std::floor(hv);
   // (Trotter systems: the records of neighbour lists of <= 8, k_sqa_csr's prefetching form)
   if ((sqa_csr ? Dp <= 8 : Dp <= 32) && n < 65536) {
-    if ((rc = build_rec(false, &c->d_rec)) ||
-        (!sqa_csr && fits8 && (rc = build_rec(true, &c->d_rec8)))) {
+    if ((rc = build_rec(false, &c->d_rec)) || (fits8 && (rc = build_rec(true, &c->d_rec8)))) {
       delete c;
       return rc;
     }
@@ -1272,8 +1279,8 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   // INT32 sweeps of couplings within int8: the values also packed four per
   // word (word (q, pos) holds entries 4q .. 4q+3 of position pos, padding 0),
   // for the dot4 field form of the CSR kernels
-  if (!sqa_csr) {
-    if (fits8) {
+  if (!sqa_csr && c->ftype == FT_I32) {
+    if (fitsv) {
       std::vector<uint32_t> p8((size_t)(Dp / 4 * n), 0u);
       for (int64_t d = 0; d < Dp; ++d)
         for (int64_t pos = 0; pos < n; ++pos) {

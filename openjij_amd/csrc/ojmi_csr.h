@@ -4,6 +4,8 @@
 #pragma once
 #include "ojmi_kernels.h"
 
+#include <type_traits>
+
 namespace ojmi {
 
 // Neighbour lists are padded to DMAX entries (zero couplings on column 0), so
@@ -12,10 +14,12 @@ namespace ojmi {
 // wave-per-replica kernel, 32 in the workgroup form; wider lists are loaded
 // at the start of their own pass).  A zero coupling adds +-0
 // to the field, which changes no decision.
-// V8 (INT32 fields, couplings within int8): the values of a pass are
-// DMAX / 4 words of four packed int8, and the field is formed by dot4
-// instructions on the gathered neighbour spins packed the same way (exact
-// integer sums, so the same field as the per-neighbour adds).
+// V8 (integer couplings within int8 and integer fields h, any field type):
+// the values of a pass are DMAX / 4 words of four packed int8, and the field
+// is formed by dot4 instructions on the gathered neighbour spins packed the
+// same way, then converted to the field type.  The sum of such integers is
+// exact in int32, and so is every partial sum of the per-neighbour adds in
+// FP32 (|h| <= 2^20, so below 2^24) and FP64: the same field, bit for bit.
 // C16 (spin indices below 65536): the columns are DMAX / 2 words of two
 // 16-bit indices, loaded from the position records below.  Both forms cut
 // the vector memory work of a pass, which bounds the workgroup form (texture
@@ -25,6 +29,7 @@ struct CsrPass {
   int i;        // spin index of this lane's position
   bool valid;   // position inside the level
   FT hv;
+  int32_t h8;   // V8: the field h as an integer
   int col[C16 ? 1 : DMAX];
   uint32_t cw[C16 ? DMAX / 2 : 1];
   FT val[V8 ? 1 : DMAX];
@@ -41,14 +46,15 @@ __device__ __forceinline__ int csr_col(const CsrPass<FT, DMAX, V8, C16>& P, int 
 // words of position pos are 16-byte chunks k at rec[k * n + pos], so each
 // chunk of a pass is one coalesced 16-byte load per lane.  Words: the
 // 16-bit column pairs [0, DMAX/2), the values (int8 quadruples if V8, else
-// FT), the spin index, the field h (FT); zero padding to a multiple of 4.
+// FT), the spin index, the field h (int32 if V8, else FT); zero padding to a
+// multiple of 4.
 template <typename FT, int DMAX, bool V8>
 struct CsrRec {
   static constexpr int kCols = 0;
   static constexpr int kVals = DMAX / 2;
   static constexpr int kIdx = kVals + (V8 ? DMAX / 4 : DMAX * (int)sizeof(FT) / 4);
   static constexpr int kH = kIdx + 1;
-  static constexpr int kWords = (kH + (int)sizeof(FT) / 4 + 3) / 4 * 4;
+  static constexpr int kWords = (kH + (V8 ? 1 : (int)sizeof(FT) / 4) + 3) / 4 * 4;
 };
 
 // The level-order arrays as buffer resources: a pass's loads then share one
@@ -127,7 +133,11 @@ __device__ __forceinline__ void csr_load_pass_at(const CsrArgs& a, const CsrBufs
 #pragma unroll
       for (int q = 0; q < DMAX / 2; ++q) P.cw[q] = wd[L::kCols + q];
       P.i = (int)wd[L::kIdx];
-      if constexpr (sizeof(FT) == 8) {
+      if constexpr (V8) {
+        P.h8 = (int32_t)wd[L::kH];
+#pragma unroll
+        for (int q = 0; q < DMAX / 4; ++q) P.v8[q] = wd[L::kVals + q];
+      } else if constexpr (sizeof(FT) == 8) {
         P.hv = __builtin_bit_cast(FT, (uint64_t)wd[L::kH] | ((uint64_t)wd[L::kH + 1] << 32));
 #pragma unroll
         for (int d = 0; d < DMAX; ++d)
@@ -135,17 +145,15 @@ __device__ __forceinline__ void csr_load_pass_at(const CsrArgs& a, const CsrBufs
               FT, (uint64_t)wd[L::kVals + 2 * d] | ((uint64_t)wd[L::kVals + 2 * d + 1] << 32));
       } else {
         P.hv = __builtin_bit_cast(FT, wd[L::kH]);
-        if constexpr (V8) {
 #pragma unroll
-          for (int q = 0; q < DMAX / 4; ++q) P.v8[q] = wd[L::kVals + q];
-        } else {
-#pragma unroll
-          for (int d = 0; d < DMAX; ++d) P.val[d] = __builtin_bit_cast(FT, wd[L::kVals + d]);
-        }
+        for (int d = 0; d < DMAX; ++d) P.val[d] = __builtin_bit_cast(FT, wd[L::kVals + d]);
       }
     } else {
+      static_assert(!V8 || std::is_same<FT, int32_t>::value,
+                    "packed word arrays (val8) hold INT32 systems");
       P.i = a.order[pc];
-      P.hv = reinterpret_cast<const FT*>(a.h)[pc];
+      if constexpr (V8) P.h8 = reinterpret_cast<const int32_t*>(a.h)[pc];
+      else P.hv = reinterpret_cast<const FT*>(a.h)[pc];
 #pragma unroll
       for (int d = 0; d < DMAX; ++d) P.col[d] = a.ell_col[(int64_t)d * n + pc];
       if constexpr (V8) {
@@ -171,22 +179,23 @@ __device__ __forceinline__ void csr_load_pass(const CsrArgs& a, const CsrBufs& B
 // The field of a pass's spin from the slice's int8 spins s (stored order).
 template <typename FT, int DMAX, bool V8, bool C16>
 __device__ __forceinline__ FT csr_field(const CsrPass<FT, DMAX, V8, C16>& P, const int8_t* s) {
-  FT f = P.hv;
   if constexpr (V8) {
-    static_assert(sizeof(FT) == 4 && (FT)0.5 == (FT)0, "packed values need INT32 fields");
+    int32_t acc = P.h8;
 #pragma unroll
     for (int q = 0; q < DMAX / 4; ++q) {
       const uint32_t w = (uint32_t)(uint8_t)s[csr_col(P, 4 * q)] |
                          ((uint32_t)(uint8_t)s[csr_col(P, 4 * q + 1)] << 8) |
                          ((uint32_t)(uint8_t)s[csr_col(P, 4 * q + 2)] << 16) |
                          ((uint32_t)(uint8_t)s[csr_col(P, 4 * q + 3)] << 24);
-      f = __builtin_amdgcn_sdot4((int)w, (int)P.v8[q], f, false);
+      acc = __builtin_amdgcn_sdot4((int)w, (int)P.v8[q], acc, false);
     }
+    return (FT)acc;
   } else {
+    FT f = P.hv;
 #pragma unroll
     for (int d = 0; d < DMAX; ++d) f = (s[csr_col(P, d)] < 0) ? f - P.val[d] : f + P.val[d];
+    return f;
   }
-  return f;
 }
 
 }  // namespace ojmi

@@ -78,6 +78,7 @@ struct CsrArgs {
   // position pos at k * n + pos; words: 16-bit column pairs, the values (int8
   // quadruples when val8 is set, else the field type), the spin index, h
   const uint4* rec = nullptr;
+  int pack8 = 0;  // rec holds int8 value quadruples and integer h (integer couplings within int8)
 };
 
 // Workgroup-shared dense sweep (k_shared.hip): J in natural row layout
