@@ -157,29 +157,31 @@ __device__ __forceinline__ TT accept_log(uint32_t u, int rule) {
 // double) equals dE < ceil(theta) for an integer dE, so a sweep can hold
 // T = ceil(theta), clamped to int32 (dE = +-2F is even, never INT32_MAX), as
 // an int32.  T comes from a hardware base-2 logarithm in single precision
-// (v_log_f32, within 1 ulp; the single-precision uniform is within 2^-24 of
-// U) whenever theta's bracket theta' -+ eps, eps = |nb| ((|l1| + |l2|)
-// 2^-16 + 2^-19) with l1, l2 the natural logarithms taken, holds one value of
-// ceil: eps exceeds that error more than a hundredfold.  Otherwise (about 1
-// word in 10^4 at |theta| ~ 10) the double logarithm decides, so T is
-// ceil(theta) exactly in every case.
+// (v_log_f32, within 1 ulp; the single-precision uniform is within 2^-23 of
+// U) and single-precision products whenever the bracket th -+ eps,
+// eps = |c| (m 2^-16 + 2^-18) with c = ln2 * nb and m the sum of the
+// logarithms' magnitudes, holds one value of ceil below 2^24: eps is at least
+// four times the error of th and far above its rounding, so theta lies
+// strictly inside the same unit interval.  Otherwise (about 1 word in 10^4 at
+// |theta| ~ 10, and every |theta| >= 2^24) the double logarithm decides, so T
+// is ceil(theta) exactly in every case.
 __device__ __forceinline__ int32_t clamp_ceil_i32(double x) {
   const double c = __builtin_ceil(x);
   return c >= 2147483647.0 ? INT32_MAX : (c <= -2147483648.0 ? INT32_MIN : (int32_t)c);
 }
 __device__ __forceinline__ int32_t int_threshold(uint32_t u, int rule, double nb) {
-  constexpr double kLn2 = 0x1.62e42fefa39efp-1;
+  const float c = (float)(0x1.62e42fefa39efp-1 * nb);
   const float l2a = __builtin_amdgcn_logf(__builtin_fmaf((float)u, 0x1p-32f, 0x1p-33f));
-  double x = (double)l2a, mag = __builtin_fabs((double)l2a);
+  float x = l2a, mag = __builtin_fabsf(l2a);
   if (rule == RULE_HEAT_BATH) {
     const float l2b = __builtin_amdgcn_logf(__builtin_fmaf((float)~u, 0x1p-32f, 0x1p-33f));
-    x -= (double)l2b;
-    mag += __builtin_fabs((double)l2b);
+    x -= l2b;
+    mag += __builtin_fabsf(l2b);
   }
-  const double th = x * kLn2 * nb;
-  const double eps = __builtin_fabs(nb) * __builtin_fma(mag * kLn2, 0x1p-16, 0x1p-19);
-  const int32_t lo = clamp_ceil_i32(th - eps), hi = clamp_ceil_i32(th + eps);
-  if (lo == hi) return lo;
+  const float th = x * c;
+  const float eps = __builtin_fabsf(c) * __builtin_fmaf(mag, 0x1p-16f, 0x1p-18f);
+  const float lo = __builtin_ceilf(th - eps), hi = __builtin_ceilf(th + eps);
+  if (lo == hi && __builtin_fabsf(lo) <= 16777216.0f) return (int32_t)lo;
   return clamp_ceil_i32(accept_log<double>(u, rule) * nb);
 }
 
