@@ -458,8 +458,11 @@ def run_config(cfg, steps, warmup, with_cpu, with_plain, opts, sweeps=None, repl
     if with_plain:
         # same seed and initial state as the last timed step of the fast path:
         # with integer couplings the two paths must end in identical states
+        # plain: FP64 fields and FP64 couplings (sparse systems keep their
+        # FP64 values rather than int8 quadruples)
+        popts = {"csr_pack8": 0} if W["kind"] in ("csr", "sqa_csr") else None
         pw, pinfos, e_plain, _, pinfo, _ = time_path(W, W["plain"], 1, 0,
-                                                     1000 + warmup + steps - 1)
+                                                     1000 + warmup + steps - 1, popts)
         p_step = float(pw.mean())
         plain = dict(value=attempts / p_step, ms_per_step=p_step * 1e3,
                      kernel_ms=float(np.mean([i["last_kernel_ms"] for i in pinfos])),
