@@ -196,10 +196,12 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *                    <= 8: 1 (default) loads each 64-spin pass's neighbour
  *                    lists as per-position 16-byte records; 0 loads one word
  *                    per column and value; results are identical
- *   "csr_pack8":     sparse INT32 contexts whose couplings lie in -127..127:
- *                    1 (default) reads the couplings packed four per word as
- *                    int8 and forms fields with 4-way integer dot products;
- *                    0 reads int32 couplings; results are identical
+ *   "csr_pack8":     sparse contexts whose couplings are integers in
+ *                    -127..127 (FP32 / FP64 fields: and whose h are integers
+ *                    of magnitude <= 2^20): 1 (default) reads the couplings
+ *                    packed four per word as int8 and forms fields with 4-way
+ *                    integer dot products (exact); 0 reads them in the field
+ *                    type; results are identical
  *   "sqa_rows_lds":  dense Trotter contexts of <= 512 spins: 1 (default)
  *                    keeps the first 1, 2 or 4 slots (64 columns each) of
  *                    every coupling row in LDS, as many as fit beside the
