@@ -380,6 +380,33 @@ def test_csr_int_couplings_packed_and_plain_bitexact(deg, vmax, waves):
         np.testing.assert_array_equal(got, want, err_msg=f"csr_pack8={pack}")
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("deg,big_h", [(5, False), (5, True), (12, False)])
+def test_csr_real_fields_with_int8_couplings(prec, deg, big_h):
+    """FP32 / FP64 fields over integer couplings within int8 and integer h
+    take the int8 dot-product form (csr_pack8 = 1, the default); an h of
+    2^21 keeps the field-type values (an FP32 partial sum could then round).
+    Both forms equal the oracle's sequential sweep in the field type."""
+    n = 700
+    rp, ci, v = int_sparse_csr(n, deg, 127, 0xF8 + deg)
+    if big_h:  # the field of spin 3: the entries (3, n) and (n, 3)
+        v = v.copy()
+        v[(np.repeat(np.arange(n + 1), np.diff(rp)) == 3) & (ci == n)] = 2.0 ** 21 + 1
+        v[(np.repeat(np.arange(n + 1), np.diff(rp)) == n) & (ci == 3)] = 2.0 ** 21 + 1
+    beta, mcs = sched(8, 0.01, 0.5)
+    R = 6
+    init = H.random_pm1(R, n, deg + 1)
+    want = orc.replay_csr(prec, rp, ci, v, n, init, 909, 0, orc.expand_classical(beta, mcs))
+    for pack in (1, 0):
+        with capi.Context.csr(n, rp, ci, v, R, precision=PREC[prec]) as ctx:
+            assert ctx.info()["kernel"] == capi.KERNEL_CSR_LEVEL
+            ctx.set_option("csr_pack8", pack)
+            ctx.set_spins(init)
+            ctx.run(beta, mcs, 909, 0)
+            got = ctx.get_spins()
+        np.testing.assert_array_equal(got, want, err_msg=f"csr_pack8={pack}")
+
+
 @pytest.mark.parametrize("prec,waves", [("fp32", 4), ("fp32", 2), ("fp32", 1), ("fp64", 0)])
 def test_csr_degree_32_real_couplings(prec, waves):
     """Neighbour lists padded to 32: FP32 fields run on the workgroup form
