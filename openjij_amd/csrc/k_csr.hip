@@ -47,7 +47,9 @@ struct WgThresh<int32_t> {
   typedef int32_t type;
 };
 
-template <typename FT, int DMAX, bool V8>
+// REC: the pass loads the per-position records (rec; V8: int8 quadruples);
+// else word arrays (V8: INT32 systems' val8).
+template <typename FT, int DMAX, bool V8, bool REC>
 __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   typedef typename CsrThresh<FT>::type TT;
   constexpr bool PREFETCH = DMAX <= 16;
@@ -69,8 +71,9 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
   const uint32_t slo = (uint32_t)a.seed, shi = (uint32_t)(a.seed >> 32);
 
   const CsrBufs CB{};  // unused: plain global loads
-  CsrPass<FT, DMAX, V8> cur;
-  if constexpr (PREFETCH) csr_load_pass<FT, DMAX, false, V8>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
+  CsrPass<FT, DMAX, V8, REC> cur;
+  if constexpr (PREFETCH)
+    csr_load_pass<FT, DMAX, false, V8, false, REC>(a, CB, a.lvl_ptr, 0, 0, lane, cur);
   uint64_t nflip = 0;  // wave-uniform
   for (int t = 0; t < a.T; ++t) {
     const uint32_t tg = (uint32_t)(a.t0 + (uint64_t)t);
@@ -98,11 +101,11 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
         nlv = 0;
         nbase = 0;
       }
-      CsrPass<FT, DMAX, V8> nxt;
+      CsrPass<FT, DMAX, V8, REC> nxt;
       if constexpr (PREFETCH) {
-        csr_load_pass<FT, DMAX, false, V8>(a, CB, a.lvl_ptr, nlv, nbase, lane, nxt);
+        csr_load_pass<FT, DMAX, false, V8, false, REC>(a, CB, a.lvl_ptr, nlv, nbase, lane, nxt);
       } else {
-        csr_load_pass<FT, DMAX, false, V8>(a, CB, a.lvl_ptr, lv, base, lane, cur);
+        csr_load_pass<FT, DMAX, false, V8, false, REC>(a, CB, a.lvl_ptr, lv, base, lane, cur);
       }
       bool flipped = false;
       if (cur.valid) {
@@ -245,14 +248,14 @@ __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
 // that holds the maximum degree D.
 int csr_padded_width(int D) { return D <= 8 ? 8 : (D <= 16 ? 16 : (D <= 32 ? 32 : 64)); }
 
-template <typename FT, bool V8>
+template <typename FT, bool V8, bool REC>
 static hipError_t launch_csr_ft(const CsrArgs& a, dim3 grid, dim3 block, size_t lds,
                                 hipStream_t st) {
   switch (a.D) {
-    case 8: hipLaunchKernelGGL((k_csr_level<FT, 8, V8>), grid, block, lds, st, a); break;
-    case 16: hipLaunchKernelGGL((k_csr_level<FT, 16, V8>), grid, block, lds, st, a); break;
-    case 32: hipLaunchKernelGGL((k_csr_level<FT, 32, V8>), grid, block, lds, st, a); break;
-    case 64: hipLaunchKernelGGL((k_csr_level<FT, 64, V8>), grid, block, lds, st, a); break;
+    case 8: hipLaunchKernelGGL((k_csr_level<FT, 8, V8, REC>), grid, block, lds, st, a); break;
+    case 16: hipLaunchKernelGGL((k_csr_level<FT, 16, V8, REC>), grid, block, lds, st, a); break;
+    case 32: hipLaunchKernelGGL((k_csr_level<FT, 32, V8, REC>), grid, block, lds, st, a); break;
+    case 64: hipLaunchKernelGGL((k_csr_level<FT, 64, V8, REC>), grid, block, lds, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -332,10 +335,21 @@ hipError_t launch_csr_level(int ftype, const CsrArgs& a, int wpb, int waves, hip
   const int ng = (a.n + 3) >> 2;
   const size_t lds = (size_t)wpb * ((size_t)ng * 20);
   dim3 grid((a.R + wpb - 1) / wpb), block(64 * wpb);
-  if (ftype == FT_F64) return launch_csr_ft<double, false>(a, grid, block, lds, st);
-  if (ftype == FT_F32) return launch_csr_ft<float, false>(a, grid, block, lds, st);
-  if (ftype == FT_I32 && a.val8) return launch_csr_ft<int32_t, true>(a, grid, block, lds, st);
-  if (ftype == FT_I32) return launch_csr_ft<int32_t, false>(a, grid, block, lds, st);
+  if (a.rec && a.pack8) {
+    if (ftype == FT_F64) return launch_csr_ft<double, true, true>(a, grid, block, lds, st);
+    if (ftype == FT_F32) return launch_csr_ft<float, true, true>(a, grid, block, lds, st);
+    if (ftype == FT_I32) return launch_csr_ft<int32_t, true, true>(a, grid, block, lds, st);
+  } else if (a.rec) {
+    if (ftype == FT_F64) return launch_csr_ft<double, false, true>(a, grid, block, lds, st);
+    if (ftype == FT_F32) return launch_csr_ft<float, false, true>(a, grid, block, lds, st);
+    if (ftype == FT_I32) return launch_csr_ft<int32_t, false, true>(a, grid, block, lds, st);
+  } else {
+    if (ftype == FT_F64) return launch_csr_ft<double, false, false>(a, grid, block, lds, st);
+    if (ftype == FT_F32) return launch_csr_ft<float, false, false>(a, grid, block, lds, st);
+    if (ftype == FT_I32 && a.val8)
+      return launch_csr_ft<int32_t, true, false>(a, grid, block, lds, st);
+    if (ftype == FT_I32) return launch_csr_ft<int32_t, false, false>(a, grid, block, lds, st);
+  }
   return hipErrorInvalidValue;
 }
 

@@ -1210,8 +1210,8 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
     delete c;
     return fail(OJMI_EDEVICE, "hipMemcpy of CSR values failed");
   }
-  // position records of the workgroup form (its LDS holds at most 32768
-  // spins, so 16-bit columns): per position the column pairs, the values,
+  // position records of the CSR kernels (the classical kernels hold at most
+  // 32768 spins in LDS, so 16-bit columns): per position the column pairs, the values,
   // the spin index and h, as 16-byte chunks [words/4][n] (the layout of
   // CsrRec in ojmi_csr.h); INT32 couplings within int8 get a second set with
   // the values as int8 quadruples
@@ -1270,7 +1270,7 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   if (c->ftype != FT_I32)
     for (double hv : hpos) fits8 = fits8 && std::fabs(hv) <= 1048576.0 && hv == std::floor(hv);
   // (Trotter systems: the records of neighbour lists of <= 8, k_sqa_csr's prefetching form)
-  if ((sqa_csr ? Dp <= 8 : Dp <= 32) && n < 65536) {
+  if ((sqa_csr ? Dp <= 8 : Dp <= 64) && n < 65536) {
     if ((rc = build_rec(false, &c->d_rec)) || (fits8 && (rc = build_rec(true, &c->d_rec8)))) {
       delete c;
       return rc;
