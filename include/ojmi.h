@@ -193,7 +193,8 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *   "lds_rows_wpb":  replicas per workgroup of that form (0 = by R: 1 up to
  *                    256 replicas, then up to 4)
  *   "sqa_csr_records": sparse Trotter contexts of < 65536 spins and degree
- *                    <= 8: 1 (default) loads each 64-spin pass's neighbour
+ *                    <= 16 (<= 8 for FP64 values that are not int8
+ *                    integers): 1 (default) loads each 64-spin pass's neighbour
  *                    lists as per-position 16-byte records; 0 loads one word
  *                    per column and value; results are identical
  *   "csr_pack8":     sparse contexts whose couplings are integers in

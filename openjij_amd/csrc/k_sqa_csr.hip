@@ -31,9 +31,9 @@ __host__ __device__ inline size_t sqa_csr_spin_bytes(int n, int M) {
 // STAGE: each wave writes the Philox words of its slice's attempts to its own
 // LDS region before the slice's passes (one Philox evaluation per 4 spins
 // instead of one per spin); else every attempt evaluates its own.
-// REC (neighbour lists of <= 8, n < 65536): a pass loads the per-position
+// REC (neighbour lists of <= 16, n < 65536): a pass loads the per-position
 // records (CsrRec in ojmi_csr.h, 16-byte chunks) instead of one word per
-// column, value, index and field.
+// column, value, index and field, one pass ahead also at 16 neighbours.
 // V8 (with REC): the records hold int8 coupling quadruples and integer h.
 template <typename FT, int DMAX, bool STAGE, bool REC, bool V8>
 __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(1024) void k_sqa_csr(CsrArgs a) {
         auto attempt = [&](int i, FT f) -> bool {
           return decide(i, f, s[i], (int)su[i] + (int)sd[i], word(i));
         };
-        if constexpr (DMAX <= 8) {
+        if constexpr (DMAX <= 8 || (REC && DMAX <= 16)) {
           // the next pass's neighbour lists in flight while this one runs
           int lv = 0, base = 0;
           const CsrBufs CB = REC ? CsrBufs{} : csr_bufs<FT>(a);
@@ -190,12 +190,14 @@ static hipError_t launch_sqa_csr_d(const CsrArgs& a, bool stage, dim3 grid, dim3
   const size_t lds = stage ? sqa_csr_staged_lds_bytes(a.n, a.M) : sqa_csr_lds_bytes(a.n, a.M);
   auto kern = stage ? k_sqa_csr<FT, DMAX, true, false, false>
                     : k_sqa_csr<FT, DMAX, false, false, false>;
-  if constexpr (DMAX <= 8) {
+  if constexpr (DMAX <= 16) {
     if (a.rec && a.pack8)
       kern = stage ? k_sqa_csr<FT, DMAX, true, true, true> : k_sqa_csr<FT, DMAX, false, true, true>;
-    else if (a.rec)
-      kern = stage ? k_sqa_csr<FT, DMAX, true, true, false>
-                   : k_sqa_csr<FT, DMAX, false, true, false>;
+    else if constexpr (DMAX <= 8 || sizeof(FT) == 4) {  // (FP64 values of 16: registers)
+      if (a.rec)
+        kern = stage ? k_sqa_csr<FT, DMAX, true, true, false>
+                     : k_sqa_csr<FT, DMAX, false, true, false>;
+    }
   }
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

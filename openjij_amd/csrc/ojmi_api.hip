@@ -1269,8 +1269,8 @@ int ojmi_create_csr(int64_t n, const int64_t* rowptr, const int64_t* col, const 
   bool fits8 = fitsv;
   if (c->ftype != FT_I32)
     for (double hv : hpos) fits8 = fits8 && std::fabs(hv) <= 1048576.0 && hv == std::floor(hv);
-  // (Trotter systems: the records of neighbour lists of <= 8, k_sqa_csr's prefetching form)
-  if ((sqa_csr ? Dp <= 8 : Dp <= 64) && n < 65536) {
+  // (Trotter systems: the records of neighbour lists of <= 16, k_sqa_csr's prefetching form)
+  if ((sqa_csr ? Dp <= 16 : Dp <= 64) && n < 65536) {
     if ((rc = build_rec(false, &c->d_rec)) || (fits8 && (rc = build_rec(true, &c->d_rec8)))) {
       delete c;
       return rc;

@@ -95,11 +95,14 @@ def test_sqa_csr_bitexact(kind, size, M, prec, R, L):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_sqa_csr_records_and_word_loads_agree(prec):
-    """Neighbour lists of <= 8 load as per-position 16-byte records (the
-    default) or one word per column and value (sqa_csr_records = 0): the
-    same spins, equal to the oracle's replay."""
-    n, rp, ci, va = graph("ea3d", (6,), 0x5C0D)
+@pytest.mark.parametrize("kind,size", [("ea3d", (6,)), ("gauss", (300, 8))])
+def test_sqa_csr_records_and_word_loads_agree(prec, kind, size):
+    """Neighbour lists of <= 16 load as per-position 16-byte records (the
+    default; EA3D's +-1 couplings as int8 quadruples) or one word per column
+    and value (sqa_csr_records = 0): the same spins, equal to the oracle's
+    replay."""
+    n, rp, ci, va = graph(kind, size, 0x5C0D)
+    assert 8 < max_degree(rp, ci, n) <= 16 if kind == "gauss" else True
     M, R, L, gamma = 6, 5, 8, 1.5
     beta, s, mcs, a, B = trotter_schedule(L, gamma, M)
     init = np.stack([H.random_pm1(M, n, 29 + r) for r in range(R)])
