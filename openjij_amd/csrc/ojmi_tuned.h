@@ -14,7 +14,7 @@ struct CsrShape {
   int waves;
 };
 constexpr CsrShape kCsr[] = {
-    {0, 362, 8},
+    {0, 362, 4},
     {0, 724, 4},
     {0, 1448, 4},
     {0, 2896, 4},
