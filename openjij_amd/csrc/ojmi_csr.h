@@ -87,14 +87,17 @@ __device__ __forceinline__ FT csr_ld(__amdgpu_buffer_rsrc_t r, int voff, int sof
 }
 
 // lp: the level pointers (global memory, or a copy in LDS); lv is
-// wave-uniform, and so is the level end read from it.  EMPTY: base may lie
-// past the level's end (the workgroup form's waves beyond a short level);
-// such a pass issues no loads (config B: 72.4 -> 67.2 ms; the check costs
-// the wave-per-replica kernel, which has no empty passes, 7%).  BUF: loads through
-// the buffer resources B (the Trotter kernel: EA3D L = 16, M = 8, R = 1024
-// 4.7 ms against 5.4); else plain global loads (the classical kernels: the
-// workgroup form, with a barrier per level, took 111 ms against 88 on EA3D
-// L = 16 with buffer loads).
+// wave-uniform, and so is the level end read from it.
+// EMPTY: base may lie past the level's end (the workgroup form's waves
+// beyond a short level); such a pass issues no loads (EA3D L = 16,
+// R = 1024: 72.4 -> 67.2 ms; the check costs the wave-per-replica kernel,
+// which has no empty passes, 7%).
+// C16: the pass loads its per-position records (CsrRec), 16 bytes per load.
+// BUF: word loads through the buffer resources B (the Trotter kernel's form
+// without records: EA3D L = 16, M = 8, R = 1024 4.7 ms against 5.4 with
+// plain loads); else plain global loads (the classical workgroup form, with
+// a barrier per level, took 111 ms against 88 on EA3D L = 16 with buffer
+// loads).
 // csr_load_pass_at: the same with the level's end b1 (wave-uniform) given.
 template <typename FT, int DMAX, bool BUF, bool V8 = false, bool EMPTY = false, bool C16 = false>
 __device__ __forceinline__ void csr_load_pass_at(const CsrArgs& a, const CsrBufs& B, int b1,
