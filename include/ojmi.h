@@ -214,8 +214,9 @@ int ojmi_batch_beta_stats(ojmi_ctx* ctx, const int8_t* spins, double* out);
  *                    phase of 4 sweeps ahead, 0 = none, -1 (default) = 2 while
  *                    the replicas number at most twice the CUs, else none;
  *                    info.waves_per_replica reports 1 + helpers of the last run
- *   "csr_waves":     CSR contexts of degree <= 16: waves sharing one replica's
- *                    sweep (1, 2, 4, 8; 0 = chosen by the replica count)
+ *   "csr_waves":     CSR contexts of degree <= 32 (16 with FP64 fields): waves
+ *                    sharing one replica's sweep (1, 2, 3, 4, 8; 0 = chosen by
+ *                    the replica count)
  *   "update_method": acceptance rule of classical contexts (dense, CSR,
  *                    batch): 0 = METROPOLIS (default), 1 = HEAT_BATH, the
  *                    reference's algorithm::UpdateMethod values

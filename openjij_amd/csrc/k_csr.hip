@@ -294,6 +294,7 @@ static hipError_t launch_csr_wg_t(const CsrArgs& a, hipStream_t st) {
 template <typename FT, bool V8 = false>
 static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
   if (W == 2) return launch_csr_wg_t<FT, 2, V8>(a, st);
+  if (W == 3) return launch_csr_wg_t<FT, 3, V8>(a, st);
   if (W == 4) return launch_csr_wg_t<FT, 4, V8>(a, st);
   if (W == 8) return launch_csr_wg_t<FT, 8, V8>(a, st);
   return hipErrorInvalidValue;
@@ -301,7 +302,7 @@ static hipError_t launch_csr_wg_ft(const CsrArgs& a, int W, hipStream_t st) {
 
 // Waves per replica for the workgroup form (1 = the wave-per-replica kernel):
 // several waves split a level's passes, so each SIMD holds more waves and each
-// wave fewer serial passes.  `requested` (1, 2, 4 or 8) overrides the choice;
+// wave fewer serial passes.  `requested` (1, 2, 3, 4 or 8) overrides the choice;
 // 0 chooses by R.  Graphs of degree > 32 (> 16 with FP64 fields: two passes'
 // records must fit the registers), or whose thresholds, spins and level
 // pointers exceed one workgroup's LDS, take the wave-per-replica kernel.

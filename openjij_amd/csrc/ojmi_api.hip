@@ -1706,8 +1706,8 @@ int ojmi_set_option(ojmi_ctx* c, const char* key, int64_t value) {
     return OJMI_OK;
   }
   if (k == "csr_waves") {
-    if (!(value == 0 || value == 1 || value == 2 || value == 4 || value == 8))
-      return fail(OJMI_EINVAL, "csr_waves must be 0, 1, 2, 4 or 8");
+    if (!(value == 0 || value == 1 || value == 2 || value == 3 || value == 4 || value == 8))
+      return fail(OJMI_EINVAL, "csr_waves must be 0, 1, 2, 3, 4 or 8");
     c->csr_waves = (int)value;
     return OJMI_OK;
   }

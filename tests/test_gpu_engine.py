@@ -265,7 +265,7 @@ def test_csr_energies_from_level_order_arrays(prec):
                 assert abs(e - ex) <= 1e-12 * (1 + abs(ex)), (r, e, ex)
 
 
-@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+@pytest.mark.parametrize("waves", [1, 2, 3, 4, 8])
 def test_csr_waves_option_bitexact(waves):
     n, rp, ci, v = H.ea3d_csr(5, 0xB0C5)
     beta, mcs = sched(12, 0.1, 4.0)
