@@ -38,12 +38,18 @@ struct CsrThresh<float> {
 
 // Thresholds of the workgroup form: INT32 fields hold ceil(theta) as int32
 // (int_threshold), half the LDS of a double and an integer compare.
-template <typename FT>
+// FP64 fields over int8 integer couplings (V8) have integer-valued dE too,
+// so they take the INT32 thresholds as well (the same double theta).
+template <typename FT, bool V8>
 struct WgThresh {
   typedef typename CsrThresh<FT>::type type;
 };
+template <bool V8>
+struct WgThresh<int32_t, V8> {
+  typedef int32_t type;
+};
 template <>
-struct WgThresh<int32_t> {
+struct WgThresh<double, true> {
   typedef int32_t type;
 };
 
@@ -137,7 +143,7 @@ __global__ __launch_bounds__(256) void k_csr_level(CsrArgs a) {
 // passes inside a level does not matter.
 template <typename FT, int DMAX, int W, bool V8>
 __global__ __launch_bounds__(64 * W) void k_csr_wg(CsrArgs a) {
-  typedef typename WgThresh<FT>::type TT;
+  typedef typename WgThresh<FT, V8>::type TT;
   typedef typename CsrThresh<FT>::type NB;  // the sweep's scale and exact thresholds
   static_assert(DMAX <= 16 || (DMAX == 32 && sizeof(FT) == 4),
                 "the workgroup form holds two passes' records in registers");
@@ -271,7 +277,7 @@ constexpr size_t CSR_WG_LDS_MAX = 160 * 1024;  // LDS of one gfx950 workgroup
 
 template <typename FT, int DMAX, int W, bool V8>
 static hipError_t launch_csr_wg_d(const CsrArgs& a, hipStream_t st) {
-  const size_t lds = csr_wg_lds_bytes(a.n, a.nlev, sizeof(typename WgThresh<FT>::type));
+  const size_t lds = csr_wg_lds_bytes(a.n, a.nlev, sizeof(typename WgThresh<FT, V8>::type));
   if (lds > CSR_WG_LDS_MAX) return hipErrorInvalidValue;
   auto kern = k_csr_wg<FT, DMAX, W, V8>;
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),

@@ -430,28 +430,30 @@ def test_csr_degree_32_real_couplings(prec, waves):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("prec", ["int32", "fp64"])
 @pytest.mark.parametrize("rule", [0, 1])
 @pytest.mark.parametrize("waves", [4, 1])
-def test_csr_int_thresholds_at_extreme_betas(rule, waves):
+def test_csr_int_thresholds_at_extreme_betas(rule, waves, prec):
     """The workgroup form holds INT32 thresholds as ceil(log-factor(U) / -beta)
     in int32: single-precision logarithms decide unless the bracket straddles
     an integer, and values beyond int32 clamp.  Betas from 1e-9 (thresholds
     ~1e10, clamped) to 1e9, with many near-integer thresholds at beta = 1 and
     0.5, replay the oracle's double comparisons bit for bit, for the
-    Metropolis and heat-bath rules."""
+    Metropolis and heat-bath rules; FP64 fields over these int8 couplings
+    take the same integer thresholds."""
     n, rp, ci, v = H.ea3d_csr(6, 0x7E5 + rule)
     v = v * 3.0
     beta = np.array([1e-9, 1e-6, 0.001, 0.25, 0.5, 1.0, 1.0, 2.0, 1e3, 1e9])
     mcs = np.full(len(beta), 2, np.int64)
     R = 40
     init = H.random_pm1(R, n, 77)
-    with capi.Context.csr(n, rp, ci, v, R, precision=capi.PREC_INT32) as ctx:
+    with capi.Context.csr(n, rp, ci, v, R, precision=PREC[prec]) as ctx:
         ctx.set_option("csr_waves", waves)
         ctx.set_option("update_method", rule)
         ctx.set_spins(init)
         ctx.run(beta, mcs, 4321, 0)
         got = ctx.get_spins()
-    want = orc.replay_csr("int32", rp, ci, v, n, init, 4321, 0,
+    want = orc.replay_csr(prec, rp, ci, v, n, init, 4321, 0,
                           orc.expand_classical(beta, mcs), rule=rule)
     np.testing.assert_array_equal(got, want)
 
